@@ -1,0 +1,360 @@
+// C ABI of the hdgx library (include/hdgx.h): contexts, master-table upload,
+// argument validation and dispatch to the per-(physics, nd, p) kernels.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/hdgx.h"
+#include "hdgx_kernels.h"
+
+#include "hdgx_launch.h"
+
+using namespace hdgx;
+
+struct MasterBuf {
+  void* dev = nullptr;
+  DevMaster tab{};
+  int nd = 0, porder = 0, npv = 0, npf = 0, nfe = 0, ngv = 0, ngf = 0;
+};
+
+struct hdgx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int prec = HDGX_PREC_FP64;
+  std::string err;
+  std::map<uint64_t, MasterBuf> masters;
+  void* ws = nullptr;
+  size_t ws_bytes = 0;
+};
+
+namespace {
+
+int fail(hdgx_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, call)                                                       \
+  do {                                                                           \
+    hipError_t _e = (call);                                                      \
+    if (_e != hipSuccess) return fail(ctx, HDGX_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
+  } while (0)
+
+uint64_t fnv(uint64_t h, const void* p, size_t n) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  return h;
+}
+
+// Upload (once per distinct table content) the master tables plus the
+// node-on-face lookup loc(v, j) derived from perm.
+int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
+  if (!m || !m->shapvt || !m->shapvg || !m->shapvgdotshapvl || !m->shapft || !m->shapfg || !m->shapfgdotshapfc ||
+      !m->shapmv || !m->shapmf || !m->perm || !m->permgeom)
+    return fail(ctx, HDGX_ERR_ARG, "master: null table");
+  const int nd = m->nd, npv = m->npv, npf = m->npf, nfe = m->nfe, ngv = m->ngv, ngf = m->ngf;
+  if (nd < 2 || nd > 3 || npv < 1 || npf < 1 || nfe != nd + 1 || ngv < 1 || ngf < 1)
+    return fail(ctx, HDGX_ERR_ARG, "master: bad sizes");
+  const size_t n_vt = (size_t)ngv * npv * (nd + 1), n_vg = (size_t)npv * ngv * (nd + 1);
+  const size_t n_vv = (size_t)npv * npv * ngv * (nd + 1), n_ft = (size_t)ngf * npf * nd;
+  const size_t n_fg = (size_t)npf * ngf * nd, n_ff = (size_t)npf * npf * ngf * nd;
+  const size_t n_mv = n_vt, n_mf = n_ft, n_perm = (size_t)npf * nfe, n_loc = (size_t)npv * nfe;
+  uint64_t h = 1469598103934665603ull;
+  int dims[7] = {nd, m->porder, npv, npf, nfe, ngv, ngf};
+  h = fnv(h, dims, sizeof(dims));
+  h = fnv(h, m->shapvt, n_vt * 8); h = fnv(h, m->shapvg, n_vg * 8); h = fnv(h, m->shapvgdotshapvl, n_vv * 8);
+  h = fnv(h, m->shapft, n_ft * 8); h = fnv(h, m->shapfg, n_fg * 8); h = fnv(h, m->shapfgdotshapfc, n_ff * 8);
+  h = fnv(h, m->shapmv, n_mv * 8); h = fnv(h, m->shapmf, n_mf * 8);
+  h = fnv(h, m->perm, n_perm * 4); h = fnv(h, m->permgeom, n_perm * 4);
+  auto it = ctx->masters.find(h);
+  if (it != ctx->masters.end()) { *out = &it->second; return HDGX_OK; }
+
+  std::vector<int32_t> loc(n_loc, -1);
+  for (int j = 0; j < nfe; ++j)
+    for (int a = 0; a < npf; ++a) {
+      const int v = m->perm[a + npf * j];
+      if (v < 0 || v >= npv) return fail(ctx, HDGX_ERR_ARG, "master: perm out of range");
+      loc[v + (size_t)npv * j] = a;
+    }
+  for (size_t i = 0; i < n_perm; ++i)
+    if (m->permgeom[i] < 0 || m->permgeom[i] >= npv) return fail(ctx, HDGX_ERR_ARG, "master: permgeom out of range");
+  const size_t nd64 = n_vt + n_vg + n_vv + n_ft + n_fg + n_ff + n_mv + n_mf;
+  const size_t bytes = nd64 * 8 + (2 * n_perm + n_loc) * 4;
+  std::vector<char> host(bytes);
+  char* p = host.data();
+  auto put = [&](const void* src, size_t n) { std::memcpy(p, src, n); char* r = p; p += n; return r; };
+  std::vector<char*> offs;
+  offs.push_back(put(m->shapvt, n_vt * 8)); offs.push_back(put(m->shapvg, n_vg * 8));
+  offs.push_back(put(m->shapvgdotshapvl, n_vv * 8)); offs.push_back(put(m->shapft, n_ft * 8));
+  offs.push_back(put(m->shapfg, n_fg * 8)); offs.push_back(put(m->shapfgdotshapfc, n_ff * 8));
+  offs.push_back(put(m->shapmv, n_mv * 8)); offs.push_back(put(m->shapmf, n_mf * 8));
+  offs.push_back(put(m->perm, n_perm * 4)); offs.push_back(put(m->permgeom, n_perm * 4));
+  offs.push_back(put(loc.data(), n_loc * 4));
+  MasterBuf mb;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  HIP_TRY(ctx, hipMalloc(&mb.dev, bytes));
+  HIP_TRY(ctx, hipMemcpy(mb.dev, host.data(), bytes, hipMemcpyHostToDevice));
+  char* base = static_cast<char*>(mb.dev);
+  auto dp = [&](int i) { return base + (offs[i] - host.data()); };
+  mb.tab.shapvt = (const double*)dp(0); mb.tab.shapvg = (const double*)dp(1); mb.tab.svv = (const double*)dp(2);
+  mb.tab.shapft = (const double*)dp(3); mb.tab.shapfg = (const double*)dp(4); mb.tab.sff = (const double*)dp(5);
+  mb.tab.shapmv = (const double*)dp(6); mb.tab.shapmf = (const double*)dp(7);
+  mb.tab.perm = (const int32_t*)dp(8); mb.tab.permgeom = (const int32_t*)dp(9); mb.tab.loc = (const int32_t*)dp(10);
+  mb.nd = nd; mb.porder = m->porder; mb.npv = npv; mb.npf = npf; mb.nfe = nfe; mb.ngv = ngv; mb.ngf = ngf;
+  auto res = ctx->masters.emplace(h, mb);
+  *out = &res.first->second;
+  return HDGX_OK;
+}
+
+// Physics plugins and boundary models the kernels implement.
+int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) {
+  if (!a) return fail(ctx, HDGX_ERR_ARG, "app: null");
+  const int nd = m->nd;
+  if (a->nd != nd) return fail(ctx, HDGX_ERR_ARG, "app.nd != master.nd");
+  if (a->ncu != nd + 2 || a->nch != a->ncu) return fail(ctx, HDGX_ERR_ARG, "app: ncu must be nd+2");
+  if (a->getdqdg != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "app: only getdqdg = 1 (nco = nc) is implemented");
+  if (a->physics == HDGX_PHYS_EULER) {
+    if (a->nc != a->ncu || a->localsolve != 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "Euler: u-only (localsolve 0, nc = ncu)");
+    if (a->nparam < 3) return fail(ctx, HDGX_ERR_ARG, "Euler: param needs {gam, eps, ..., tau}");
+    if (nd == 2 && a->param[1] > 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "Euler 2D far field with eps > 0");
+  } else if (a->physics == HDGX_PHYS_NS) {
+    if (nd != 2 || a->nc != 12 || a->localsolve != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "NS: 2D with u,q (nc 12, localsolve 1)");
+    if (a->nparam < 6) return fail(ctx, HDGX_ERR_ARG, "NS: param needs {gam, eps, Re, Pr, Minf, tau}");
+  } else {
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "unknown physics");
+  }
+  if (a->ncd < nd || a->ncd > nd + 1) return fail(ctx, HDGX_ERR_ARG, "app: ncd must be nd or nd+1");
+  if (a->av_col >= a->ncd || (a->av_col >= 0 && a->av_col < nd)) return fail(ctx, HDGX_ERR_ARG, "app: av_col");
+  if (a->nbc < 0 || a->nbc > 8) return fail(ctx, HDGX_ERR_ARG, "app: nbc");
+  std::memset(d, 0, sizeof(*d));
+  d->gam = a->param[0];
+  d->eps = a->param[1];
+  if (a->physics == HDGX_PHYS_NS) { d->Re = a->param[2]; d->Pr = a->param[3]; }
+  d->tau = a->param[a->nparam - 1];
+  d->fc_u = a->fc_u;
+  d->fc_q = a->fc_q;
+  d->tdep = a->tdep;
+  d->av_col = a->av_col;
+  d->ncd = a->ncd;
+  d->nbc = a->nbc;
+  for (int i = 0; i < a->nbc; ++i) {
+    const int ib = a->bcm[i];
+    const bool ok = ib == 1 || (ib == 2 && a->physics == HDGX_PHYS_EULER && nd == 2);
+    if (!ok) return fail(ctx, HDGX_ERR_UNSUPPORTED, "boundary model " + std::to_string(ib) + " not implemented");
+    d->bcm[i] = ib;
+    for (int c = 0; c < a->ncu; ++c) d->bcs[i * 5 + c] = a->bcs[i * HDGX_MAX_PARAM + c];
+  }
+  if (a->localsolve && !(a->fc_q != 0.0)) return fail(ctx, HDGX_ERR_ARG, "app: fc_q must be nonzero");
+  return HDGX_OK;
+}
+
+// ---------------------------------------------------------------- dispatch
+template <class C>
+int launch_quad(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const QuadArgs& qa) {
+  if (mb.npv != C::NPV || mb.npf != C::NPF || mb.ngv != C::NGV || mb.ngf != C::NGF)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled pgauss = 2p configuration");
+  if (qa.ne == 0) return HDGX_OK;
+  HIP_TRY(ctx, quad_launch<C>(ctx->stream, qa, mb.tab, ap));
+  return HDGX_OK;
+}
+
+template <class C>
+int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa) {
+  if (sa.ne == 0) return HDGX_OK;
+  HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa));
+  return HDGX_OK;
+}
+
+// The configurations built into the library.
+template <class F>
+int with_cfg(int phys, int nd, int p, F&& f) {
+#define HDGX_CASE(PH, ND, P) if (phys == PH && nd == ND && p == P) return f(Cfg<PH, ND, P>{});
+  HDGX_FOR_EACH_CFG(HDGX_CASE)
+#undef HDGX_CASE
+  return -1;
+}
+
+int porder_of(int nd, int npv) {
+  for (int p = 1; p <= 6; ++p) {
+    int n = nd == 2 ? (p + 1) * (p + 2) / 2 : (p + 1) * (p + 2) * (p + 3) / 6;
+    if (n == npv) return p;
+  }
+  return -1;
+}
+
+int ensure_ws(hdgx_ctx* ctx, size_t bytes) {
+  if (ctx->ws_bytes >= bytes) return HDGX_OK;
+  if (ctx->ws) HIP_TRY(ctx, hipFree(ctx->ws));
+  ctx->ws = nullptr;
+  ctx->ws_bytes = 0;
+  HIP_TRY(ctx, hipMalloc(&ctx->ws, bytes));
+  ctx->ws_bytes = bytes;
+  return HDGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdgx_create(hdgx_ctx** ctx, int device) {
+  if (!ctx) return HDGX_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HDGX_ERR_DEVICE;
+  auto* c = new hdgx_ctx();
+  c->device = device;
+  *ctx = c;
+  return HDGX_OK;
+}
+
+int hdgx_destroy(hdgx_ctx* ctx) {
+  if (!ctx) return HDGX_ERR_ARG;
+  (void)hipSetDevice(ctx->device);
+  for (auto& kv : ctx->masters) (void)hipFree(kv.second.dev);
+  if (ctx->ws) (void)hipFree(ctx->ws);
+  delete ctx;
+  return HDGX_OK;
+}
+
+int hdgx_set_stream(hdgx_ctx* ctx, void* stream) {
+  if (!ctx) return HDGX_ERR_ARG;
+  ctx->stream = static_cast<hipStream_t>(stream);
+  return HDGX_OK;
+}
+
+int hdgx_set_precision(hdgx_ctx* ctx, int precision) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (precision != HDGX_PREC_FP64 && precision != HDGX_PREC_FP32_IR) return fail(ctx, HDGX_ERR_ARG, "precision");
+  ctx->prec = precision;
+  return HDGX_OK;
+}
+
+const char* hdgx_last_error(hdgx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int hdgx_elemental_matrices(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                            const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
+                            const double* sh, double* BD, double* F, double* GK, double* H, double* Ru,
+                            double* Rh, double* M, double* C, double* E, double* Rq) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  DevApp ap;
+  if ((rc = make_app(ctx, master, app, &ap))) return rc;
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !BD || !F || !GK || !H || !Ru || !Rh))
+    return fail(ctx, HDGX_ERR_ARG, "null array");
+  if (ne > 0 && app->localsolve && (!M || !C || !E || !Rq)) return fail(ctx, HDGX_ERR_ARG, "localsolve needs M, C, E, Rq");
+  if (ne > 0 && (app->localsolve || app->tdep) && !sh && app->tdep) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
+  QuadArgs qa{ne, dgnodes, bf, udg, uh, sh, BD, F, GK, H, Ru, Rh, M, C, E, Rq, nullptr};
+  const int p = porder_of(master->nd, master->npv);
+  rc = with_cfg(app->physics, master->nd, p, [&](auto cfg) { return launch_quad<decltype(cfg)>(ctx, *mb, ap, qa); });
+  if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+  return rc;
+}
+
+int hdgx_schur_u(hdgx_ctx* ctx, int npv, int ncu, int nc, int ndf, int64_t ne, const double* BD, const double* F,
+                 const double* GK, const double* H, const double* Ru, const double* Rh, double* dudg,
+                 double* dudg_duh, double* ae, double* fe, int32_t* status) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0 || nc != ncu) return fail(ctx, HDGX_ERR_ARG, "schur_u: nc must equal ncu");
+  if (ne > 0 && (!BD || !F || !GK || !H || !Ru || !Rh || !dudg || !dudg_duh || !ae || !fe))
+    return fail(ctx, HDGX_ERR_ARG, "null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int nd = ncu - 2;
+  const int p = porder_of(nd, npv);
+  SchurArgs sa{ne, nullptr, nullptr, nullptr, BD, F, GK, H, nullptr, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec};
+  int rc = with_cfg(0, nd, p, [&](auto cfg) {
+    using Cc = decltype(cfg);
+    if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: ndf does not match a simplex of this order");
+    return launch_schur<Cc>(ctx, sa);
+  });
+  if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: no kernel for these sizes");
+  return rc;
+}
+
+int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int64_t ne, const double* M,
+                  const double* C, const double* E, const double* BD, const double* F, const double* GK,
+                  const double* H, const double* Rq, const double* Ru, const double* Rh, double* dudg,
+                  double* dudg_duh, double* ae, double* fe, int32_t* status) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0 || nc != ncu * (nd + 1)) return fail(ctx, HDGX_ERR_ARG, "schur_uq: nc must be ncu*(nd+1)");
+  if (ne > 0 && (!M || !C || !E || !BD || !F || !GK || !H || !Rq || !Ru || !Rh || !dudg || !dudg_duh || !ae || !fe))
+    return fail(ctx, HDGX_ERR_ARG, "null array");
+  if (nd != 2 || ncu != 4) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: 2D with ncu = 4 is built");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int p = porder_of(nd, npv);
+  SchurArgs sa{ne, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec};
+  int rc = with_cfg(1, nd, p, [&](auto cfg) {
+    using Cc = decltype(cfg);
+    if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: ndf does not match");
+    return launch_schur<Cc>(ctx, sa);
+  });
+  if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: no kernel for these sizes");
+  return rc;
+}
+
+int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                   const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
+                   const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh, int32_t* status) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !ae || !fe || !dudg || !dudg_duh))
+    return fail(ctx, HDGX_ERR_ARG, "null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  DevApp ap;
+  if ((rc = make_app(ctx, master, app, &ap))) return rc;
+  if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
+  const int nd = master->nd, npv = master->npv, ncu = app->ncu, nc = app->nc;
+  const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
+  const int p = porder_of(nd, npv);
+  const bool uq = app->localsolve != 0;
+  // per-element workspace for the Schur inputs
+  const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
+                     (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(ne, (int64_t)((size_t)6 << 30) / (int64_t)(per * 8)));
+  if (ne == 0) return HDGX_OK;
+  if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
+  for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
+    const int64_t n = std::min(chunk, ne - e0);
+    double* w = static_cast<double*>(ctx->ws);
+    double* BD = w; w += (size_t)nu * ncol * n;
+    double* F = w; w += (size_t)nu * nh * n;
+    double* GK = w; w += (size_t)nh * ncol * n;
+    double* H = w; w += (size_t)nh * nh * n;
+    double* Ru = w; w += (size_t)nu * n;
+    double* Rh = w; w += (size_t)nh * n;
+    double *M = nullptr, *C = nullptr, *E = nullptr, *Rq = nullptr;
+    if (uq) {
+      M = w; w += (size_t)npv * npv * n;
+      C = w; w += (size_t)npv * npv * nd * n;
+      E = w; w += (size_t)npv * ndf * nd * n;
+      Rq = w; w += (size_t)npv * ncu * nd * n;
+    }
+    QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe, udg + (size_t)e0 * npv * nc,
+                uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr, BD, F, GK, H, Ru, Rh, M, C, E, Rq,
+                nullptr};
+    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg + (size_t)e0 * ncol, dudg_duh + (size_t)e0 * ncol * nh,
+                 ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh, status ? status + e0 : nullptr, ctx->prec};
+    rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
+      using Cc = decltype(cfg);
+      int r = launch_quad<Cc>(ctx, *mb, ap, qa);
+      if (r) return r;
+      return launch_schur<Cc>(ctx, sa);
+    });
+    if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+    if (rc) return rc;
+  }
+  return HDGX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,39 @@
+// One (physics, nd, p) configuration of the hdgx kernels; built once per
+// configuration with -DHDGX_PHYS= -DHDGX_ND= -DHDGX_P=.
+#include <hip/hip_runtime.h>
+
+#include "hdgx_launch.h"
+#include "quad_kernel.hip"
+#include "schur_kernel.hip"
+
+namespace hdgx {
+
+template <class C>
+constexpr int chunk_cols() {
+  // columns of [rhs0 | trace rhs] solved per register chunk: <= 16, balanced
+  constexpr int nrhs = 1 + C::NH;
+  constexpr int nch = (nrhs + 15) / 16;
+  return (nrhs + nch - 1) / nch;
+}
+
+template <class C>
+hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
+  hipLaunchKernelGGL(quad_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t schur_launch(hipStream_t s, const SchurArgs& a) {
+  constexpr int CH = chunk_cols<C>();
+  if (a.prec == 1)
+    hipLaunchKernelGGL((schur_kernel<C, CH, float>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((schur_kernel<C, CH, double>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+
+using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
+template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
+template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&);
+
+}  // namespace hdgx

@@ -1,0 +1,75 @@
+// Kernel configurations and argument blocks shared by the hdgx kernels and
+// the host dispatch.
+#pragma once
+
+#include <stdint.h>
+
+namespace hdgx {
+
+constexpr int tri_npts(int pg) { return pg <= 1 ? 1 : pg == 2 ? 3 : pg == 3 ? 4 : pg == 4 ? 6 : pg == 5 ? 7 : pg == 6 ? 12 : pg == 7 ? 13 : 16; }
+constexpr int tet_npts(int pg) { return pg <= 2 ? 4 : pg == 3 ? 5 : pg == 4 ? 11 : pg == 5 ? 15 : 24; }
+
+// Element configuration fixed at compile time: physics, dimension and order.
+// Quadrature is the reference default pgauss = 2p (master/mkmaster.m:10-12).
+template <int PHYS_, int ND_, int P_>
+struct Cfg {
+  static constexpr int PHYS = PHYS_, ND = ND_, P = P_;
+  static constexpr int NPV = (ND == 2) ? (P + 1) * (P + 2) / 2 : (P + 1) * (P + 2) * (P + 3) / 6;
+  static constexpr int NPF = (ND == 2) ? P + 1 : (P + 1) * (P + 2) / 2;
+  static constexpr int NFE = ND + 1;
+  static constexpr int NDF = NPF * NFE;
+  static constexpr int NGV = (ND == 2) ? tri_npts(2 * P) : tet_npts(2 * P);
+  static constexpr int NGF = (ND == 2) ? P + 1 : tri_npts(2 * P);
+  static constexpr int NCU = ND + 2;
+  static constexpr int NC = (PHYS == 1) ? NCU * (ND + 1) : NCU;
+  static constexpr bool UQ = (PHYS == 1);
+  static constexpr int NU = NPV * NCU;       // local u unknowns
+  static constexpr int NH = NCU * NDF;       // trace unknowns
+  static constexpr int NCOL = NPV * NC;      // columns of BD / GK, rows of DUDG_DUH
+  static constexpr int NR = NGV * (ND + 1);  // volume quadrature rows (value + derivatives)
+};
+
+// Master element tables on the device (column-major, reference shapes).
+struct DevMaster {
+  const double* shapvt;           // (ngv, npv, nd+1)
+  const double* shapvg;           // (npv, ngv, nd+1)
+  const double* svv;              // shapvgdotshapvl (npv*npv, ngv, nd+1)
+  const double* shapft;           // (ngf, npf, nd)
+  const double* shapfg;           // (npf, ngf, nd)
+  const double* sff;              // shapfgdotshapfc (npf*npf, ngf, nd)
+  const double* shapmv;           // (ngv, npv, nd+1)
+  const double* shapmf;           // (ngf, npf, nd)
+  const int32_t* perm;            // (npf, nfe)
+  const int32_t* permgeom;        // (npf, nfe)
+  const int32_t* loc;             // (npv, nfe): position of volume node v on face j, or -1
+};
+
+// Physics parameters passed by value.
+struct DevApp {
+  double gam, eps, Re, Pr, tau;
+  double fc_u, fc_q;
+  int32_t tdep, av_col, ncd, nbc;
+  int32_t bcm[8];
+  double bcs[8 * 5];               // bcs[i*5 + c]
+};
+
+struct QuadArgs {
+  int64_t ne;
+  const double* dgnodes;
+  const int32_t* bf;
+  const double* udg;
+  const double* uh;
+  const double* sh;
+  double *BD, *F, *GK, *H, *Ru, *Rh, *M, *C, *E, *Rq;
+  int32_t* status;
+};
+
+struct SchurArgs {
+  int64_t ne;
+  const double *M, *C, *E, *BD, *F, *GK, *H, *Rq, *Ru, *Rh;
+  double *dudg, *dudg_duh, *ae, *fe;
+  int32_t* status;
+  int32_t prec;                    // 0 fp64, 1 fp32 + fp64 refinement
+};
+
+}  // namespace hdgx

@@ -1,0 +1,28 @@
+// Per-configuration launchers.  Each configuration is compiled in its own
+// translation unit (hdgx_inst.hip with HDGX_PHYS/HDGX_ND/HDGX_P defined), so
+// the kernels build in parallel; the API layer only sees these declarations.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "hdgx_kernels.h"
+
+// The configurations built into the library: (physics, nd, p).
+#define HDGX_FOR_EACH_CFG(X) \
+  X(0, 2, 1) X(0, 2, 2) X(0, 2, 3) X(0, 2, 4) \
+  X(1, 2, 1) X(1, 2, 2) X(1, 2, 3) X(1, 2, 4) \
+  X(0, 3, 1) X(0, 3, 2)
+
+namespace hdgx {
+
+template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
+template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a);
+
+#define HDGX_DECLARE(PH, ND, P)                                                                          \
+  extern template hipError_t quad_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
+                                                         const DevApp&);                                  \
+  extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&);
+HDGX_FOR_EACH_CFG(HDGX_DECLARE)
+#undef HDGX_DECLARE
+
+}  // namespace hdgx

@@ -1,0 +1,232 @@
+"""Host runtime over the hdgx C ABI (include/hdgx.h), with torch for device
+memory and streams.
+
+The product path runs only on the HIP library: if libhdgx.so is missing or
+no GPU is visible, every call raises -- there is no CPU fallback.
+
+Functions mirror the reference's operator interface:
+``hdg_elemental`` (kernel/hdg_elemental.m:1), ``schur_u`` / ``schur_uq`` (the MEX
+files kernel/schur_u.c, kernel/schur_uq.c) and ``elemental_matrices`` (the
+volint + faceint outputs).  Arrays keep the reference's MATLAB shapes; on the
+device they are flat column-major torch tensors (``Dev``).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .app import App, CApp, MasterC
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libhdgx.so")
+
+PREC_FP64 = 0
+PREC_FP32_IR = 1
+
+_lib = None
+
+
+class HdgxError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise HdgxError(f"{LIB} is not built; run python -m hdg_amd.build")
+        L = ctypes.CDLL(LIB)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+        L.hdgx_create.argtypes = [ctypes.POINTER(vp), i32]
+        L.hdgx_destroy.argtypes = [vp]
+        L.hdgx_set_stream.argtypes = [vp, vp]
+        L.hdgx_set_precision.argtypes = [vp, i32]
+        L.hdgx_last_error.argtypes = [vp]
+        L.hdgx_last_error.restype = ctypes.c_char_p
+        L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
+        L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
+        L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
+        L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
+        _lib = L
+    return _lib
+
+
+EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
+           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq"]
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise HdgxError("no GPU visible: the hdgx path runs only on the device")
+    return torch
+
+
+class Dev:
+    """A device array: flat column-major float64/int32 torch tensor + MATLAB shape."""
+
+    def __init__(self, t, shape):
+        self.t = t
+        self.shape = tuple(int(s) for s in shape)
+
+    @property
+    def ptr(self):
+        return ctypes.c_void_p(self.t.data_ptr())
+
+    def numpy(self):
+        return self.t.detach().cpu().numpy().reshape(self.shape, order="F")
+
+    @staticmethod
+    def from_numpy(a, dtype=None, device="cuda"):
+        torch = _torch()
+        a = np.asarray(a)
+        if dtype is not None:
+            a = a.astype(dtype)
+        flat = np.ascontiguousarray(a.ravel(order="F"))
+        return Dev(torch.from_numpy(flat).to(device), a.shape)
+
+    @staticmethod
+    def empty(shape, dtype="float64", device="cuda"):
+        torch = _torch()
+        n = int(np.prod(shape)) if len(shape) else 1
+        td = torch.float64 if dtype == "float64" else torch.int32
+        return Dev(torch.empty(max(n, 1), dtype=td, device=device), shape)
+
+
+def _dev(a, dtype="float64"):
+    if a is None:
+        return None
+    if isinstance(a, Dev):
+        return a
+    return Dev.from_numpy(a, np.float64 if dtype == "float64" else np.int32)
+
+
+def _p(a):
+    return None if a is None else a.ptr
+
+
+class Context:
+    """One hdgx context (per host thread / stream) on a device."""
+
+    def __init__(self, device=0, precision=PREC_FP64):
+        torch = _torch()
+        self.device = device
+        h = ctypes.c_void_p()
+        rc = lib().hdgx_create(ctypes.byref(h), device)
+        if rc:
+            raise HdgxError(f"hdgx_create failed ({rc})")
+        self.h = h
+        self._masters = {}
+        self.set_precision(precision)
+        self.bind_stream(torch.cuda.current_stream(device))
+
+    def bind_stream(self, stream):
+        self.stream = stream
+        lib().hdgx_set_stream(self.h, ctypes.c_void_p(stream.cuda_stream))
+
+    def set_precision(self, precision):
+        self.precision = precision
+        self._check(lib().hdgx_set_precision(self.h, precision))
+
+    def _check(self, rc):
+        if rc:
+            msg = lib().hdgx_last_error(self.h).decode()
+            raise HdgxError(f"hdgx error {rc}: {msg}")
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().hdgx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def master_c(self, master):
+        key = id(master)
+        if key not in self._masters:
+            self._masters[key] = (master, MasterC(master))
+        return self._masters[key][1]
+
+    # ------------------------------------------------------------ operators
+    def elemental(self, master, app, dgnodes, bf, udg, uh, sh=None, out=None):
+        """[AE, FE, DUDG, DUDG_DUH, status] = hdg_elemental(master, app, dgnodes, bf, UDG, UH, SH)."""
+        ca = app.to_c() if isinstance(app, App) else app
+        mc = self.master_c(master)
+        dg, b, u, h = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh)
+        s = _dev(sh)
+        ne = dg.shape[2]
+        npv, ncu, nc = master.npv, ca.ncu, ca.nc
+        ndf = master.npf * master.nfe
+        if out is None:
+            out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+                   Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+        ae, fe, dudg, dudg_duh, st = out
+        self._check(lib().hdgx_elemental(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
+                                         h.ptr, _p(s), ae.ptr, fe.ptr, dudg.ptr, dudg_duh.ptr, st.ptr))
+        return ae, fe, dudg, dudg_duh, st
+
+    def elemental_matrices(self, master, app, dgnodes, bf, udg, uh, sh=None):
+        ca = app.to_c() if isinstance(app, App) else app
+        mc = self.master_c(master)
+        dg, b, u, h = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh)
+        s = _dev(sh)
+        ne = dg.shape[2]
+        npv, ncu, nc, nd = master.npv, ca.ncu, ca.nc, master.nd
+        ndf = master.npf * master.nfe
+        out = {"BD": Dev.empty((npv, ncu, npv, nc, ne)), "F": Dev.empty((npv, ncu, ncu, ndf, ne)),
+               "GK": Dev.empty((ncu, ndf, npv, nc, ne)), "H": Dev.empty((ncu, ndf, ncu, ndf, ne)),
+               "Ru": Dev.empty((npv, ncu, ne)), "Rh": Dev.empty((ncu, ndf, ne))}
+        if ca.localsolve:
+            out.update(M=Dev.empty((npv, npv, ne)), C=Dev.empty((npv, npv, nd, ne)),
+                       E=Dev.empty((npv, ndf, nd, ne)), Rq=Dev.empty((npv, ncu, nd, ne)))
+        ptrs = [_p(out.get(k)) for k in ("BD", "F", "GK", "H", "Ru", "Rh", "M", "C", "E", "Rq")]
+        self._check(lib().hdgx_elemental_matrices(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr,
+                                                  u.ptr, h.ptr, _p(s), *ptrs))
+        return out
+
+    def _schur_out(self, npv, ncu, nc, ndf, ne):
+        return (Dev.empty((npv, nc, ne)), Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ncu, ndf, ncu, ndf, ne)),
+                Dev.empty((ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+
+    def schur_u(self, BD, F, GK, H, Ru, Rh):
+        """[DUDG, DUDG_DUH, AE, FE, status] = schur_u(BD, F, GK, H, Ru, Rh)."""
+        BD, F, GK, H, Ru, Rh = (_dev(a) for a in (BD, F, GK, H, Ru, Rh))
+        npv, ncu, _, nc, ne = BD.shape
+        ndf = H.shape[1]
+        out = self._schur_out(npv, ncu, nc, ndf, ne)
+        self._check(lib().hdgx_schur_u(self.h, npv, ncu, nc, ndf, ne, BD.ptr, F.ptr, GK.ptr, H.ptr, Ru.ptr, Rh.ptr,
+                                       *[o.ptr for o in out]))
+        return out
+
+    def schur_uq(self, M, C, E, BD, F, GK, H, Rq, Ru, Rh):
+        """[DUDG, DUDG_DUH, AE, FE, status] = schur_uq(M, C, E, BD, F, GK, H, Rq, Ru, Rh)."""
+        M, C, E, BD, F, GK, H, Rq, Ru, Rh = (_dev(a) for a in (M, C, E, BD, F, GK, H, Rq, Ru, Rh))
+        npv, ne = M.shape[0], M.shape[2]
+        ndf, nd = E.shape[1], E.shape[2]
+        ncu, nc = BD.shape[1], BD.shape[3]
+        out = self._schur_out(npv, ncu, nc, ndf, ne)
+        self._check(lib().hdgx_schur_uq(self.h, nd, npv, ncu, nc, ndf, ne, M.ptr, C.ptr, E.ptr, BD.ptr, F.ptr,
+                                        GK.ptr, H.ptr, Rq.ptr, Ru.ptr, Rh.ptr, *[o.ptr for o in out]))
+        return out
+
+
+_default = {}
+
+
+def context(device=0):
+    if device not in _default:
+        _default[device] = Context(device)
+    return _default[device]
+
+
+def hdg_elemental(master, app, dgnodes, bf, UDG, UH, SH=None, precision=PREC_FP64, device=0):
+    """Host-array convenience form of the elemental operator: uploads, runs on
+    the GPU and returns numpy arrays (AE, FE, DUDG, DUDG_DUH, status)."""
+    ctx = context(device)
+    ctx.set_precision(precision)
+    outs = ctx.elemental(master, app, dgnodes, bf, UDG, UH, SH)
+    ctx.stream.synchronize()
+    return tuple(o.numpy() for o in outs)

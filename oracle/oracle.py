@@ -12,21 +12,32 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+LIB_NATIVE = os.path.join(HERE, "liboracle_native.so")
 
 _lib = None
 
 
-def build(force=False):
-    if force or not os.path.exists(LIB):
-        subprocess.check_call(["make", "-s", "-C", HERE, "-B" if force else "liboracle.so"])
+def build(force=False, native=False):
+    target = "native" if native else "all"
+    if force or not os.path.exists(LIB_NATIVE if native else LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []) + [target])
 
 
-def lib():
+def use_native():
+    """Switch to the -march=native build (for CPU timing on this host)."""
+    global _lib
+    build(native=True)
+    _lib = None
+    return lib(LIB_NATIVE)
+
+
+def lib(path=None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB):
+        path = path or LIB
+        if not os.path.exists(path):
             build()
-        L = ctypes.CDLL(LIB)
+        L = ctypes.CDLL(path)
         vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
         L.oracle_elemental.argtypes = [vp, vp, i64] + [vp] * 10 + [i32]
         L.oracle_elemental_matrices.argtypes = [vp, vp, i64] + [vp] * 15 + [i32]

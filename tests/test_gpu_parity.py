@@ -1,0 +1,115 @@
+"""GPU parity: the HIP path (through the C ABI) against the oracle on the same
+seeded inputs.  FP64 bar 1e-12 normwise relative per output array; FP32 +
+FP64-refinement bar 1e-10 (the error reached is printed per array)."""
+import numpy as np
+import pytest
+
+from tests.conftest import has_gpu
+from tests.gpu_util import perturb, relerr
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
+
+from hdg_amd.problems import euler2d_square, euler3d_cube, ns2d_square  # noqa: E402
+
+CASES = {
+    "euler2d_p1": lambda: euler2d_square(3, 1, seed=21, alpha=0.3),
+    "euler2d_p2": lambda: euler2d_square(4, 2, seed=22),
+    "euler2d_p3": lambda: euler2d_square(3, 3, seed=23, alpha=0.3),
+    "euler2d_p4": lambda: euler2d_square(3, 4, seed=24),
+    "ns2d_p1": lambda: ns2d_square(3, 1, seed=25, av_frac=0.3),
+    "ns2d_p2": lambda: ns2d_square(3, 2, seed=26, av_frac=0.3),
+    "ns2d_p3": lambda: ns2d_square(4, 3, seed=27, av_frac=0.3),
+    "ns2d_p4": lambda: ns2d_square(3, 4, seed=28, av_frac=0.3),
+    "euler3d_p1": lambda: euler3d_cube(2, 1, seed=29, alpha=0.3),
+    "euler3d_p2": lambda: euler3d_cube(2, 2, seed=30, alpha=0.3),
+}
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hdg_amd.runtime import Context
+    return Context(0)
+
+
+def _inputs(case):
+    P = perturb(CASES[case](), 5)
+    return P
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_element_matrices(ctx, oracle, case):
+    P = _inputs(case)
+    print(f"[gpu] element matrices {case} ne={P.ne}", flush=True)
+    ref = oracle.elemental_matrices(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    got = ctx.elemental_matrices(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    ctx.stream.synchronize()
+    for k, v in ref.items():
+        err = relerr(got[k].numpy(), v)
+        assert err < 1e-12, (k, err)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_schur_boundary(ctx, oracle, case):
+    """hdgx_schur_u / hdgx_schur_uq on the oracle's element matrices."""
+    P = _inputs(case)
+    print(f"[gpu] schur {case} ne={P.ne}", flush=True)
+    m = oracle.elemental_matrices(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    if P.app.localsolve:
+        keys = ("M", "C", "E", "BD", "F", "GK", "H", "Rq", "Ru", "Rh")
+        ref = oracle.schur_uq(*(m[k] for k in keys))
+        got = ctx.schur_uq(*(m[k] for k in keys))
+    else:
+        keys = ("BD", "F", "GK", "H", "Ru", "Rh")
+        ref = oracle.schur_u(*(m[k] for k in keys))
+        got = ctx.schur_u(*(m[k] for k in keys))
+    ctx.stream.synchronize()
+    for name, g, r in zip(("DUDG", "DUDG_DUH", "AE", "FE"), got[:4], ref[:4]):
+        err = relerr(g.numpy(), r)
+        assert err < 1e-12, (name, err)
+    assert np.all(got[4].numpy() == 0)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+@pytest.mark.parametrize("case", list(CASES))
+def test_elemental(ctx, oracle, case, prec):
+    P = _inputs(case)
+    print(f"[gpu] elemental {case} prec={prec} ne={P.ne}", flush=True)
+    ref = oracle.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    ctx.set_precision(prec)
+    try:
+        got = ctx.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+        ctx.stream.synchronize()
+    finally:
+        ctx.set_precision(0)
+    bar = 1e-12 if prec == 0 else 1e-10
+    errs = {}
+    for name, g, r in zip(("AE", "FE", "DUDG", "DUDG_DUH"), got[:4], ref[:4]):
+        errs[name] = relerr(g.numpy(), r)
+    print(f"[gpu]   errors {errs}", flush=True)
+    assert max(errs.values()) < bar, errs
+    assert np.all(got[4].numpy() == 0)
+
+
+def test_empty_and_single(ctx, oracle):
+    P = _inputs("euler2d_p2")
+    one = (P.mesh.dgnodes[:, :, :1], P.mesh.bf[:, :1], P.udg[:, :, :1], P.uh[:, :, :1], P.sh[:, :, :1])
+    ref = oracle.elemental(P.master, P.app, *one)
+    got = ctx.elemental(P.master, P.app, *one)
+    ctx.stream.synchronize()
+    for g, r in zip(got[:4], ref[:4]):
+        assert relerr(g.numpy(), r) < 1e-12
+    from hdg_amd.runtime import Dev
+    z = (P.mesh.dgnodes[:, :, :0], P.mesh.bf[:, :0], P.udg[:, :, :0], P.uh[:, :, :0], P.sh[:, :, :0])
+    out = ctx.elemental(P.master, P.app, *z)
+    assert out[0].shape[-1] == 0
+
+
+def test_singular_element_status(ctx, oracle):
+    P = _inputs("ns2d_p2")
+    P.mesh.dgnodes[:, :2, 0] = P.mesh.dgnodes[0, :2, 0][None, :]
+    got = ctx.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    ctx.stream.synchronize()
+    st = got[4].numpy()
+    ref = oracle.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)[4]
+    assert st[0] != 0 and np.all(st[1:] == 0)
+    assert (st[0] != 0) == (ref[0] != 0)
