@@ -1,0 +1,214 @@
+"""Benchmark: element Schur solves/s of the HDG elemental hot path on one MI355X.
+
+Workload: 2D Navier-Stokes HDG elemental step (u and q, nc = 12) on a
+jittered unit square of 2 x 128 x 128 = 32768 p = 3 triangles, Re 1e4,
+Pr 0.72, M 0.3, 2% sinusoidal velocity perturbation, laplacian artificial
+viscosity 1e-3 on 5% of the elements; inputs synthetic and seeded, resident in
+HBM.  One step = one hdg_elemental pass over all elements (volint + faceint +
+schur_uq): (dgnodes, UDG, UH, SH, bf) -> (AE, FE, DUDG, DUDG_DUH).
+
+Reported: FP64 throughput (`value`) and the FP32 + FP64-refinement form beside
+it, each with its normwise relative error against the CPU FP64 oracle run on
+the same inputs; the roofline of the dominant kernel from HIP events on the
+launch stream; the oracle's CPU throughput on this host.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+METRIC = "element Schur solves/s FP64 vs FP32+IR @1e-10 rel err; p50/p99 request latency"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def relerr(x, ref):
+    n = np.linalg.norm(ref.ravel())
+    return float(np.linalg.norm((x - ref).ravel()) / (n if n > 0 else 1.0))
+
+
+def time_steps(torch, stream, fn, steps, warmup):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return ev0.elapsed_time(ev1) / steps, wall
+
+
+def kernel_times(torch, ctx, P, dev, reps):
+    """Per-kernel durations, measured with events on the launch stream: the
+    quadrature kernel through hdgx_elemental_matrices and the Schur kernel
+    through hdgx_schur_uq on its outputs (the two launches hdgx_elemental
+    makes, separated)."""
+    s = ctx.stream
+    mats = ctx.elemental_matrices(P.master, P.app, *dev)
+    keys = ("M", "C", "E", "BD", "F", "GK", "H", "Rq", "Ru", "Rh")
+    ctx.schur_uq(*(mats[k] for k in keys))
+    torch.cuda.synchronize()
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    tq = ts = 0.0
+    for _ in range(reps):
+        e[0].record(s)
+        mats = ctx.elemental_matrices(P.master, P.app, *dev)
+        e[1].record(s)
+        ctx.schur_uq(*(mats[k] for k in keys))
+        e[2].record(s)
+        torch.cuda.synchronize()
+        tq += e[0].elapsed_time(e[1])
+        ts += e[1].elapsed_time(e[2])
+    del mats
+    return tq / reps, ts / reps
+
+
+def cpu_baseline(P, nthreads):
+    from oracle import oracle as O
+    try:
+        O.use_native()
+        kind_build = "native"
+    except Exception:
+        kind_build = "portable"
+    args = (P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    O.elemental(*args, nthreads=nthreads)                     # warm-up
+    runs = []
+    out = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        out = O.elemental(*args, nthreads=nthreads)
+        runs.append(time.perf_counter() - t0)
+    t_all = float(np.median(runs))
+    # single core on a 2048-element sample (the MEX element loop is single-threaded)
+    ns = min(2048, P.ne)
+    sample = (P.master, P.app, P.mesh.dgnodes[:, :, :ns], P.mesh.bf[:, :ns], P.udg[:, :, :ns],
+              P.uh[:, :, :ns], P.sh[:, :, :ns])
+    O.elemental(*sample, nthreads=1)
+    r1 = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        O.elemental(*sample, nthreads=1)
+        r1.append(time.perf_counter() - t0)
+    t_one = float(np.median(r1))
+    return out, P.ne / t_all, ns / t_one, kind_build, sum(runs) + sum(r1)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=128, help="cells per side (2*n*n triangles)")
+    ap.add_argument("--seed", type=int, default=2024)
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the elemental step needs no communication and every box here has "
+              "one GPU; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from hdg_amd.problems import ns2d_square
+    from hdg_amd.runtime import Context, Dev, PREC_FP32_IR, PREC_FP64
+    from hdg_amd.workload import problem_work
+
+    t0 = time.perf_counter()
+    P = ns2d_square(a.n, 3, seed=a.seed)
+    W = problem_work(P)
+    log(f"[bench] workload ns2d p=3 ne={P.ne} built in {time.perf_counter() - t0:.1f}s")
+
+    ctx = Context(0)
+    stream = ctx.stream
+    dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+           Dev.from_numpy(P.uh), Dev.from_numpy(P.sh))
+    npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.ndf, P.ne
+    out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+           Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+
+    def step():
+        ctx.elemental(P.master, P.app, *dev, out=out)
+
+    res = {}
+    outputs = {}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_FP32_IR, "fp32_ir")):
+        ctx.set_precision(prec)
+        ms, wall = time_steps(torch, stream, step, a.steps, a.warmup)
+        res[name] = (ms, wall)
+        outputs[name] = [o.numpy() for o in out]
+        log(f"[bench] {name}: {ms:.3f} ms/step, {ne / ms * 1e3:.4g} elements/s (wall {wall:.2f}s)")
+    ctx.set_precision(PREC_FP64)
+
+    tq, ts = kernel_times(torch, ctx, P, dev, reps=5)
+    log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
+
+    ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    ref, cpu_rate, cpu_rate1, build_kind, cpu_secs = cpu_baseline(P, ncpu)
+    log(f"[bench] cpu oracle: {cpu_rate:.4g} elements/s on {ncpu} threads, {cpu_rate1:.4g} on 1 ({cpu_secs:.1f}s)")
+
+    names = ("AE", "FE", "DUDG", "DUDG_DUH")
+    errs = {p: {n: relerr(outputs[p][i], ref[i]) for i, n in enumerate(names)} for p in outputs}
+    status_bad = {p: int(np.count_nonzero(outputs[p][4])) for p in outputs}
+
+    ms64 = res["fp64"][0]
+    ms32 = res["fp32_ir"][0]
+    value = ne / (ms64 * 1e-3)
+    # dominant kernel roofline (FP64 element Schur + quadrature are compute bound at this size)
+    if ts >= tq:
+        kname, kflops, kms = "schur_kernel<NS2D p3>", W.schur_flops * ne, ts
+    else:
+        kname, kflops, kms = "quad_kernel<NS2D p3>", W.quad_flops * ne, tq
+    achieved = kflops / (kms * 1e-3) / 1e12
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "elements/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms64,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded jittered mesh and flow state)",
+        "config": {"workload": f"ns2d_hdg_elemental_p3_2x{a.n}x{a.n}", "elements": ne, "porder": 3,
+                   "nc": nc, "method": "u,q (schur_uq)", "parallelism": "single GPU"},
+        "fp32_ir": {"value": ne / (ms32 * 1e-3), "ms_per_step": ms32, "speedup_vs_fp64": ms64 / ms32,
+                    "max_rel_err_vs_oracle": max(errs["fp32_ir"].values()), "rel_err": errs["fp32_ir"]},
+        "fp64_rel_err_vs_oracle": errs["fp64"],
+        "status_nonzero": status_bad,
+        "step_rates": {"flops_tflops_fp64": W.flops * ne / (ms64 * 1e-3) / 1e12,
+                       "hbm_gbs_algorithmic_fp64": W.bytes * ne / (ms64 * 1e-3) / 1e9,
+                       "bytes_per_element": W.bytes, "flops_per_element": W.flops},
+        "kernels_ms": {"quad_kernel": tq, "schur_kernel": ts},
+        "roofline": {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "flops_per_element": (W.schur_flops if ts >= tq else W.quad_flops)},
+        "cpu_baseline": {"value": cpu_rate, "unit": "elements/s", "cores": ncpu, "kind": "port",
+                         "sample": f"full workload ({ne} elements), oracle FP64 C++ restatement, "
+                                   f"{build_kind} build, OpenMP over elements, median of 3 after 1 warm-up",
+                         "single_core_value": cpu_rate1, "single_core_sample": f"first {min(2048, ne)} elements"},
+        "speedup_vs_cpu": value / cpu_rate,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -164,9 +164,9 @@ int launch_quad(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const Quad
 }
 
 template <class C>
-int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa) {
+int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa, const int32_t* perm) {
   if (sa.ne == 0) return HDGX_OK;
-  HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa));
+  HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa, perm));
   return HDGX_OK;
 }
 
@@ -272,7 +272,7 @@ int hdgx_schur_u(hdgx_ctx* ctx, int npv, int ncu, int nc, int ndf, int64_t ne, c
   int rc = with_cfg(0, nd, p, [&](auto cfg) {
     using Cc = decltype(cfg);
     if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: ndf does not match a simplex of this order");
-    return launch_schur<Cc>(ctx, sa);
+    return launch_schur<Cc>(ctx, sa, nullptr);
   });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: no kernel for these sizes");
   return rc;
@@ -293,7 +293,7 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
   int rc = with_cfg(1, nd, p, [&](auto cfg) {
     using Cc = decltype(cfg);
     if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: ndf does not match");
-    return launch_schur<Cc>(ctx, sa);
+    return launch_schur<Cc>(ctx, sa, nullptr);
   });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: no kernel for these sizes");
   return rc;
@@ -349,7 +349,7 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
       using Cc = decltype(cfg);
       int r = launch_quad<Cc>(ctx, *mb, ap, qa);
       if (r) return r;
-      return launch_schur<Cc>(ctx, sa);
+      return launch_schur<Cc>(ctx, sa, mb->tab.perm);
     });
     if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
     if (rc) return rc;

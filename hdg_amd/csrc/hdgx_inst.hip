@@ -5,6 +5,7 @@
 #include "hdgx_launch.h"
 #include "quad_kernel.hip"
 #include "schur_kernel.hip"
+#include "schur_mfma.hip"
 
 namespace hdgx {
 
@@ -23,17 +24,19 @@ hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, con
 }
 
 template <class C>
-hipError_t schur_launch(hipStream_t s, const SchurArgs& a) {
+hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
   constexpr int CH = chunk_cols<C>();
   if (a.prec == 1)
     hipLaunchKernelGGL((schur_kernel<C, CH, float>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
-  else
+  else if (a.prec == 2)
     hipLaunchKernelGGL((schur_kernel<C, CH, double>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+  else
+    hipLaunchKernelGGL((schur_mfma_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
   return hipGetLastError();
 }
 
 using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
 template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
-template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&);
+template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*);
 
 }  // namespace hdgx

@@ -1,0 +1,421 @@
+// FP64 MFMA form of the batched Schur elimination (kernel/schur_u.c:42-163,
+// kernel/schur_uq.c:40-287) for gfx950.
+//
+// One wavefront per element.  The element system S = [D' | RHS] (D' = D or
+// D + B M^-1 C; RHS = [Ru, -F] or [Ru - B M^-1 Rq, B M^-1 E - F]) lives in
+// v_mfma_f64_16x16x4f64 accumulator tiles: lane l = 16g + c, register i of a
+// tile holds row 4i + g, column c.  Gauss-Jordan elimination with partial
+// pivoting runs in panels of four columns: the panel is factored one row per
+// lane (wavefront argmax for the pivot), which yields the rank-4 transform
+// S <- S + Z * S[pivot rows, :]; that update is applied to every tile right
+// of the panel with MFMA.  After the last panel row p_k of the RHS tiles holds
+// solution row k.  B M^-1 C and B M^-1 E are MFMA products whose all-zero
+// k-steps are skipped at compile time, and AE = H + GK X runs per AE row tile
+// over only the columns of GK that touch that tile's faces (GK is face
+// sparse, faceint.m:354-358).
+//
+// RHS columns are stored in the order q = 0 (du), q = 1 + jf + NDF*k for the
+// trace unknown (k, jf), so each 16-column tile meets few state components.
+#include <hip/hip_runtime.h>
+
+#include "hdgx_device.h"
+#include "hdgx_kernels.h"
+
+namespace hdgx {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+HDGX_DEV d4 mfma64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+template <class C>
+struct MfmaShape {
+  static constexpr int NU = C::NU, NH = C::NH, NRHS = 1 + C::NH;
+  static constexpr int RT = (NU + 15) / 16;            // row tiles (= D' column tiles)
+  static constexpr int R = 16 * RT;
+  static constexpr int TRALL = (NRHS + 15) / 16;       // RHS tiles
+  // RHS tiles per pass: keep the accumulators at <= 96 doubles per lane
+  static constexpr int TRB0 = (96 / (4 * RT)) - RT;
+  static constexpr int TRB = TRB0 < 1 ? 1 : (TRB0 > TRALL ? TRALL : TRB0);
+  static constexpr int NPASS = (TRALL + TRB - 1) / TRB;
+  static constexpr int CTP = RT + TRB;                 // tiles per pass
+  static constexpr int NPAN = (NU + 3) / 4;
+  static constexpr int RTH = (NH + 15) / 16;           // AE row tiles
+  static constexpr int KB = C::NPV * C::NCU * C::ND;   // columns of B (q part of BD)
+  static constexpr int KBS = (KB + 3) / 4;
+};
+
+// state component of RHS column q >= 1 and its face-node index jf
+template <class C> HDGX_DEV constexpr int rhs_k(int q) { return (q - 1) / C::NDF; }
+template <class C> HDGX_DEV constexpr int rhs_jf(int q) { return (q - 1) % C::NDF; }
+// original trace column (k + NCU*jf) of RHS column q >= 1
+template <class C> HDGX_DEV constexpr int rhs_t(int q) { return rhs_k<C>(q) + C::NCU * rhs_jf<C>(q); }
+
+// does k-step s (B columns 4s..4s+3, B column K = a + NPV*m + NPV*NCU*d) meet
+// component m for some column of the D' tile ct (D' column b + NPV*m)?
+template <class C>
+HDGX_DEV constexpr bool bmic_needed(int ct, int s) {
+  for (int kk = 4 * s; kk < 4 * s + 4 && kk < MfmaShape<C>::KB; ++kk) {
+    const int m = (kk / C::NPV) % C::NCU;
+    for (int col = 16 * ct; col < 16 * ct + 16 && col < C::NU; ++col)
+      if (col / C::NPV == m) return true;
+  }
+  return false;
+}
+
+// the same for the RHS tile with global RHS index base q0 (columns q0..q0+15)
+template <class C>
+HDGX_DEV constexpr bool bmie_needed(int q0, int s) {
+  for (int kk = 4 * s; kk < 4 * s + 4 && kk < MfmaShape<C>::KB; ++kk) {
+    const int m = (kk / C::NPV) % C::NCU;
+    for (int q = q0; q < q0 + 16 && q < 1 + C::NH; ++q)
+      if (q >= 1 && rhs_k<C>(q) == m) return true;
+  }
+  return false;
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+  using S = MfmaShape<C>;
+  constexpr int NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, NCU = C::NCU, ND = C::ND;
+  constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NC = C::NC;
+  constexpr bool UQ = C::UQ;
+  constexpr int NRHS = S::NRHS, RT = S::RT, R = S::R, TRB = S::TRB, CTP = S::CTP, RTH = S::RTH;
+  constexpr int NQ = UQ ? NPV * NCU * ND : 0;
+  constexpr int NRQ = NCU * ND, NCC = NPV * ND, NEE = NDF * ND;
+  static_assert(R <= 64, "one panel row per lane");
+
+  const int lane = threadIdx.x;
+  const int g = lane >> 4, c = lane & 15;
+  const int64_t e = blockIdx.x;
+  if (e >= A.ne) return;
+
+  __shared__ double sPan[R * 4];
+  __shared__ double sZ[R * 4];
+  __shared__ double sRows[4 * 16 * CTP];
+  __shared__ int sStep[R];
+  __shared__ int sPr[4];
+  __shared__ double sXt[NCOL * 16];
+  __shared__ double sR0[R];
+  __shared__ double sMf[UQ ? NPV * NPV : 1];
+  __shared__ double sMiRq[UQ ? NPV * NRQ : 1];
+  __shared__ double sMiC[UQ ? NPV * NCC : 1];
+  __shared__ double sMiE[UQ ? NPV * NEE : 1];
+  constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
+  __shared__ int sAeNodes[RTH * NNMAX];
+  __shared__ int sAeNN[RTH];
+
+  int st = 0;
+  const double* BDe = A.BD + e * (int64_t)NU * NCOL;
+  const double* Fe = A.F + e * (int64_t)NU * NH;
+  const double* GKe = A.GK + e * (int64_t)NH * NCOL;
+  const double* He = A.H + e * (int64_t)NH * NH;
+  double* Xe = A.dudg_duh + e * (int64_t)NCOL * NH;
+
+  // ---- node sets of the AE row tiles: the volume nodes of the faces met by
+  // rows 16*rta .. 16*rta+15 (GK is face sparse); without perm (GK given by a
+  // caller, possibly dense) every node.
+  if (lane < RTH) {
+    const int rta = lane;
+    int nn = 0;
+    if (perm) {
+      const int j0 = (16 * rta) / (NCU * NPF);
+      int j1 = (16 * rta + 15) / (NCU * NPF);
+      if (j1 > NFE - 1) j1 = NFE - 1;
+      for (int j = j0; j <= j1; ++j)
+        for (int a = 0; a < NPF; ++a) {
+          const int v = perm[a + NPF * j];
+          bool seen = false;
+          for (int q = 0; q < nn; ++q) seen |= (sAeNodes[rta * NNMAX + q] == v);
+          if (!seen) sAeNodes[rta * NNMAX + nn++] = v;
+        }
+    } else {
+      for (int v = 0; v < NPV; ++v) sAeNodes[rta * NNMAX + nn++] = v;
+    }
+    sAeNN[rta] = nn;
+  }
+
+  // ---- u,q prologue: M = U'U (dpotrf 'U'), M^-1 [Rq C E] (dpotrs)
+  if constexpr (UQ) {
+    for (int i = lane; i < NPV * NPV; i += 64) sMf[i] = A.M[e * NPV * NPV + i];
+    __syncthreads();
+    for (int j = 0; j < NPV; ++j) {
+      if (lane == 0) {
+        double v = sMf[j + NPV * j];
+        if (!(v > 0.0)) st |= 1;
+        sMf[j + NPV * j] = sqrt(v);
+      }
+      __syncthreads();
+      if (lane > j && lane < NPV) sMf[j + NPV * lane] /= sMf[j + NPV * j];
+      __syncthreads();
+      const int m = NPV - j - 1;
+      for (int idx = lane; idx < m * m; idx += 64) {
+        const int i = j + 1 + idx % m, k = j + 1 + idx / m;
+        if (i <= k) sMf[i + NPV * k] -= sMf[j + NPV * i] * sMf[j + NPV * k];
+      }
+      __syncthreads();
+    }
+    for (int col = lane; col < NRQ + NCC + NEE; col += 64) {
+      const double* src;
+      double* dst;
+      if (col < NRQ) { src = A.Rq + e * NPV * NRQ + NPV * col; dst = sMiRq + NPV * col; }
+      else if (col < NRQ + NCC) { src = A.C + e * NPV * NCC + NPV * (col - NRQ); dst = sMiC + NPV * (col - NRQ); }
+      else { src = A.E + e * NPV * NEE + NPV * (col - NRQ - NCC); dst = sMiE + NPV * (col - NRQ - NCC); }
+      double x[NPV];
+#pragma unroll
+      for (int i = 0; i < NPV; ++i) x[i] = src[i];
+#pragma unroll
+      for (int i = 0; i < NPV; ++i) {
+        double s = x[i];
+#pragma unroll
+        for (int k = 0; k < i; ++k) s -= sMf[k + NPV * i] * x[k];
+        x[i] = s / sMf[i + NPV * i];
+      }
+#pragma unroll
+      for (int i = NPV - 1; i >= 0; --i) {
+        double s = x[i];
+#pragma unroll
+        for (int k = i + 1; k < NPV; ++k) s -= sMf[i + NPV * k] * x[k];
+        x[i] = s / sMf[i + NPV * i];
+      }
+#pragma unroll
+      for (int i = 0; i < NPV; ++i) dst[i] = x[i];
+    }
+  }
+  // rhs0 = Ru (- B M^-1 Rq), one row per lane
+  if (lane < R) {
+    double v = 0.0;
+    if (lane < NU) {
+      v = A.Ru[e * NU + lane];
+      if constexpr (UQ) {
+        double s = 0.0;
+        for (int kk = 0; kk < S::KB; ++kk) s += BDe[lane + NU * (NU + kk)] * sMiRq[kk];
+        v -= s;
+      }
+    }
+    sR0[lane] = v;
+  }
+  __syncthreads();
+
+  for (int pass = 0; pass < S::NPASS; ++pass) {
+    const int qbase = 16 * TRB * pass;       // first RHS column of this pass
+    d4 acc[RT][CTP];
+    // ---- D' tiles: D, identity on the padded diagonal
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 0; ct < RT; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * rt + 4 * i + g, col = 16 * ct + c;
+          double v = 0.0;
+          if (row < NU && col < NU) v = BDe[row + NU * col];
+          else if (row == col) v = 1.0;
+          acc[rt][ct][i] = v;
+        }
+    // ---- RHS tiles: rhs0 | -F(:, trace column)
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int tb = 0; tb < TRB; ++tb)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * rt + 4 * i + g, q = qbase + 16 * tb + c;
+          double v = 0.0;
+          if (row < NU && q < NRHS) v = (q == 0) ? sR0[row] : -Fe[row + NU * rhs_t<C>(q)];
+          acc[rt][RT + tb][i] = v;
+        }
+    if constexpr (UQ) {
+      // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns
+#pragma unroll
+      for (int s = 0; s < S::KBS; ++s) {
+        const int kk = 4 * s + g;                              // B-operand row of this lane
+        const int am = kk % NPV, mm = (kk / NPV) % NCU, dm = kk / (NPV * NCU);
+        double a[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const int row = 16 * rt + c, ka = 4 * s + g;         // A fragment: row l%16, k l/16
+          a[rt] = (row < NU && ka < S::KB) ? BDe[row + NU * (NU + ka)] : 0.0;
+        }
+#pragma unroll
+        for (int ct = 0; ct < RT; ++ct) {
+          if (!bmic_needed<C>(ct, s)) continue;
+          const int col = 16 * ct + c;
+          double b = 0.0;
+          if (kk < S::KB && col < NU && col / NPV == mm) b = sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(a[rt], b, acc[rt][ct]);
+        }
+#pragma unroll
+        for (int tb = 0; tb < TRB; ++tb) {
+          // skip test uses the pass-0 layout bound; inside, the exact column decides
+          const int q = qbase + 16 * tb + c;
+          double b = 0.0;
+          if (kk < S::KB && q >= 1 && q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
+          bool any = false;
+#pragma unroll
+          for (int pp = 0; pp < S::NPASS; ++pp) any |= (pp == pass) && bmie_needed<C>(16 * TRB * pp + 16 * tb, s);
+          if (!any) continue;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[rt], b, acc[rt][RT + tb]);
+        }
+      }
+    }
+
+    // ---- Gauss-Jordan with partial pivoting, panels of four columns
+    if (lane < R) sStep[lane] = -1;
+    bool done = false;                        // lane r < R: row r already a pivot row
+    __syncthreads();
+#pragma unroll
+    for (int pn = 0; pn < S::NPAN; ++pn) {
+      const int k0 = 4 * pn, ct0 = k0 / 16, c0 = k0 % 16;
+      // panel columns -> LDS, one row per lane
+      {
+        const int t = c - c0;
+        if (t >= 0 && t < 4)
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sPan[(16 * rt + 4 * i + g) * 4 + t] = acc[rt][ct0][i];
+      }
+      __syncthreads();
+      double pr[4], z[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[lane * 4 + t] : 0.0;
+      int pvt[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        double v = -1.0;
+        if (lane < R && !done) { const double a = fabs(pr[t]); v = isnan(a) ? 1.0e308 : a; }
+        int p = lane;
+        wave_argmax(v, p);
+        if (!(v > 0.0)) st |= 2;
+        pvt[t] = p;
+        const double pv = __shfl(pr[t], p, 64);
+        z[t] = (lane == p) ? 1.0 : 0.0;
+        double ppr[4], pz[4];
+#pragma unroll
+        for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = __shfl(pr[t2], p, 64); pz[t2] = __shfl(z[t2], p, 64); }
+        if (lane == p) {
+          const double ip = 1.0 / pv;
+#pragma unroll
+          for (int t2 = 0; t2 < 4; ++t2) { pr[t2] *= ip; z[t2] *= ip; }
+          done = true;
+        } else {
+          const double m = pr[t] / pv;
+#pragma unroll
+          for (int t2 = 0; t2 < 4; ++t2) { pr[t2] -= m * ppr[t2]; z[t2] -= m * pz[t2]; }
+        }
+        if (lane == 0) { sStep[p] = k0 + t; sPr[t] = p; }
+      }
+#pragma unroll
+      for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= 1.0;
+      if (lane < R)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) sZ[lane * 4 + t] = z[t];
+      // original (pre-panel) pivot rows, columns from tile ct0 on
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int p = pvt[t];
+        const int rtp = p >> 4, ip = (p & 15) >> 2, gp = p & 3;
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            if (rt == rtp && i == ip && g == gp)
+#pragma unroll
+              for (int ct = 0; ct < CTP; ++ct)
+                if (ct >= ct0) sRows[t * 16 * CTP + 16 * ct + c] = acc[rt][ct][i];
+      }
+      __syncthreads();
+      // S += Z * S[pivot rows, :]
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt) {
+        const double a = sZ[(16 * rt + c) * 4 + g];
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct)
+          if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * 16 * CTP + 16 * ct + c], acc[rt][ct]);
+      }
+      __syncthreads();
+    }
+
+    // ---- per RHS tile: u rows by pivot order, q rows, outputs, AE/FE columns
+#pragma unroll
+    for (int tb = 0; tb < TRB; ++tb) {
+      const int q0 = qbase + 16 * tb;
+      if (q0 >= NRHS) break;
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = sStep[16 * rt + 4 * i + g];
+          if (k >= 0 && k < NU) sXt[k + NCOL * c] = acc[rt][RT + tb][i];
+        }
+      __syncthreads();
+      if constexpr (UQ) {
+        // dq = MiRq + MiC du ; dlq = MiC dlu - MiE (x) I   (schur_uq.c:146-186)
+        const int q = q0 + c;
+        for (int rr = g; rr < NQ; rr += 4) {
+          const int n = rr % NPV, m = (rr / NPV) % NCU, dd = rr / (NPV * NCU);
+          double v = 0.0;
+#pragma unroll
+          for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOL * c];
+          if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
+          else if (q < NRHS && rhs_k<C>(q) == m) v -= sMiE[n + NPV * rhs_jf<C>(q) + NPV * NDF * dd];
+          sXt[NU + rr + NCOL * c] = v;
+        }
+        __syncthreads();
+      }
+      // DUDG / DUDG_DUH columns of this tile (coalesced over rows)
+      for (int cc = 0; cc < 16; ++cc) {
+        const int q = q0 + cc;
+        if (q >= NRHS) break;
+        for (int r = lane; r < NCOL; r += 64) {
+          const double v = sXt[r + NCOL * cc];
+          if (!isfinite(v)) st |= 4;
+          if (q == 0) A.dudg[e * NCOL + r] = v;
+          else Xe[r + (int64_t)NCOL * rhs_t<C>(q)] = v;
+        }
+      }
+      // AE(:, tile) = H + GK X, FE = Rh - GK x (schur_uq.c:279-285)
+#pragma unroll
+      for (int rta = 0; rta < RTH; ++rta) {
+        d4 ae;
+        const int q = q0 + c;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * rta + 4 * i + g;
+          ae[i] = (row < NH && q >= 1 && q < NRHS) ? He[row + NH * rhs_t<C>(q)] : 0.0;
+        }
+        const int nn = sAeNN[rta];
+        const int klen = nn * NC;
+        const int row_a = 16 * rta + c;
+        for (int s0 = 0; s0 < klen; s0 += 4) {
+          const int idx = s0 + g;                     // A: k = l/16 ; B: k = l/16
+          double a = 0.0, b = 0.0;
+          if (idx < klen) {
+            const int comp = idx / nn, node = sAeNodes[rta * NNMAX + idx % nn];
+            const int K = node + NPV * comp;
+            if (row_a < NH) a = GKe[row_a + NH * K];
+            b = sXt[K + NCOL * c];
+          }
+          ae = mfma64(a, b, ae);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * rta + 4 * i + g;
+          if (row < NH && q < NRHS) {
+            double v;
+            if (q == 0) { v = A.Rh[e * NH + row] - ae[i]; A.fe[e * NH + row] = v; }
+            else { v = ae[i]; A.ae[e * (int64_t)NH * NH + row + NH * rhs_t<C>(q)] = v; }
+            if (!isfinite(v)) st |= 4;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+  st = wave_or(st);
+  if (lane == 0 && A.status) A.status[e] = st;
+}
+
+}  // namespace hdgx
