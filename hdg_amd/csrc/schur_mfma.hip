@@ -101,8 +101,10 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
   __shared__ double sMiC[UQ ? NPV * NCC : 1];
   __shared__ double sMiE[UQ ? NPV * NEE : 1];
   constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
+  constexpr int KSMAX = (NNMAX * NC + 3) / 4;            // k-steps of the AE product, upper bound
   __shared__ int sAeNodes[RTH * NNMAX];
   __shared__ int sAeNN[RTH];
+  __shared__ int sAeK[RTH * 4 * KSMAX];
 
   int st = 0;
   const double* BDe = A.BD + e * (int64_t)NU * NCOL;
@@ -132,6 +134,8 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
       for (int v = 0; v < NPV; ++v) sAeNodes[rta * NNMAX + nn++] = v;
     }
     sAeNN[rta] = nn;
+    for (int idx = 0; idx < 4 * KSMAX; ++idx)
+      sAeK[rta * 4 * KSMAX + idx] = (idx < nn * NC) ? sAeNodes[rta * NNMAX + idx % nn] + NPV * (idx / nn) : -1;
   }
 
   // ---- u,q prologue: M = U'U (dpotrf 'U'), M^-1 [Rq C E] (dpotrs)
@@ -225,38 +229,46 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
           acc[rt][RT + tb][i] = v;
         }
     if constexpr (UQ) {
-      // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns
+      // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns, A
+      // fragments (columns of B) fetched a batch of k-steps ahead
+      constexpr int BB = 5;
 #pragma unroll
-      for (int s = 0; s < S::KBS; ++s) {
-        const int kk = 4 * s + g;                              // B-operand row of this lane
-        const int am = kk % NPV, mm = (kk / NPV) % NCU, dm = kk / (NPV * NCU);
-        double a[RT];
+      for (int s0 = 0; s0 < S::KBS; s0 += BB) {
+        double a[BB][RT];
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          const int row = 16 * rt + c, ka = 4 * s + g;         // A fragment: row l%16, k l/16
-          a[rt] = (row < NU && ka < S::KB) ? BDe[row + NU * (NU + ka)] : 0.0;
-        }
+        for (int sb = 0; sb < BB; ++sb)
 #pragma unroll
-        for (int ct = 0; ct < RT; ++ct) {
-          if (!bmic_needed<C>(ct, s)) continue;
-          const int col = 16 * ct + c;
-          double b = 0.0;
-          if (kk < S::KB && col < NU && col / NPV == mm) b = sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
+          for (int rt = 0; rt < RT; ++rt) {
+            const int row = 16 * rt + c, ka = 4 * (s0 + sb) + g;     // A fragment: row l%16, k l/16
+            a[sb][rt] = (s0 + sb < S::KBS && row < NU && ka < S::KB) ? BDe[row + NU * (NU + ka)] : 0.0;
+          }
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(a[rt], b, acc[rt][ct]);
-        }
+        for (int sb = 0; sb < BB; ++sb) {
+          const int s = s0 + sb;
+          if (s >= S::KBS) break;
+          const int kk = 4 * s + g;                                  // B-operand row of this lane
+          const int am = kk % NPV, mm = (kk / NPV) % NCU, dm = kk / (NPV * NCU);
 #pragma unroll
-        for (int tb = 0; tb < TRB; ++tb) {
-          // skip test uses the pass-0 layout bound; inside, the exact column decides
-          const int q = qbase + 16 * tb + c;
-          double b = 0.0;
-          if (kk < S::KB && q >= 1 && q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
-          bool any = false;
+          for (int ct = 0; ct < RT; ++ct) {
+            if (!bmic_needed<C>(ct, s)) continue;
+            const int col = 16 * ct + c;
+            double b = 0.0;
+            if (kk < S::KB && col < NU && col / NPV == mm) b = sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
 #pragma unroll
-          for (int pp = 0; pp < S::NPASS; ++pp) any |= (pp == pass) && bmie_needed<C>(16 * TRB * pp + 16 * tb, s);
-          if (!any) continue;
+            for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(a[sb][rt], b, acc[rt][ct]);
+          }
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[rt], b, acc[rt][RT + tb]);
+          for (int tb = 0; tb < TRB; ++tb) {
+            bool any = false;
+#pragma unroll
+            for (int pp = 0; pp < S::NPASS; ++pp) any |= (pp == pass) && bmie_needed<C>(16 * TRB * pp + 16 * tb, s);
+            if (!any) continue;
+            const int q = qbase + 16 * tb + c;
+            double b = 0.0;
+            if (kk < S::KB && q >= 1 && q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[sb][rt], b, acc[rt][RT + tb]);
+          }
         }
       }
     }
@@ -376,40 +388,55 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
           else Xe[r + (int64_t)NCOL * rhs_t<C>(q)] = v;
         }
       }
-      // AE(:, tile) = H + GK X, FE = Rh - GK x (schur_uq.c:279-285)
-#pragma unroll
-      for (int rta = 0; rta < RTH; ++rta) {
-        d4 ae;
+      // AE(:, tile) = H + GK X, FE = Rh - GK x (schur_uq.c:279-285); all AE row
+      // tiles advance together, fragments fetched a batch of k-steps ahead
+      {
         const int q = q0 + c;
+        d4 ae[RTH];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = 16 * rta + 4 * i + g;
-          ae[i] = (row < NH && q >= 1 && q < NRHS) ? He[row + NH * rhs_t<C>(q)] : 0.0;
-        }
-        const int nn = sAeNN[rta];
-        const int klen = nn * NC;
-        const int row_a = 16 * rta + c;
-        for (int s0 = 0; s0 < klen; s0 += 4) {
-          const int idx = s0 + g;                     // A: k = l/16 ; B: k = l/16
-          double a = 0.0, b = 0.0;
-          if (idx < klen) {
-            const int comp = idx / nn, node = sAeNodes[rta * NNMAX + idx % nn];
-            const int K = node + NPV * comp;
-            if (row_a < NH) a = GKe[row_a + NH * K];
-            b = sXt[K + NCOL * c];
+        for (int rta = 0; rta < RTH; ++rta)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = 16 * rta + 4 * i + g;
+            ae[rta][i] = (row < NH && q >= 1 && q < NRHS) ? He[row + NH * rhs_t<C>(q)] : 0.0;
           }
-          ae = mfma64(a, b, ae);
+        constexpr int BATCH = 6;
+#pragma unroll
+        for (int s0 = 0; s0 < KSMAX; s0 += BATCH) {
+          double a[BATCH][RTH], b[BATCH][RTH];
+#pragma unroll
+          for (int sb = 0; sb < BATCH; ++sb)
+#pragma unroll
+            for (int rta = 0; rta < RTH; ++rta) {
+              a[sb][rta] = 0.0;
+              b[sb][rta] = 0.0;
+              if (s0 + sb < KSMAX) {
+                const int K = sAeK[rta * 4 * KSMAX + 4 * (s0 + sb) + g];
+                const int row_a = 16 * rta + c;
+                if (K >= 0) {
+                  if (row_a < NH) a[sb][rta] = GKe[row_a + NH * K];
+                  b[sb][rta] = sXt[K + NCOL * c];
+                }
+              }
+            }
+#pragma unroll
+          for (int sb = 0; sb < BATCH; ++sb)
+#pragma unroll
+            for (int rta = 0; rta < RTH; ++rta)
+              if (s0 + sb < KSMAX && 4 * (s0 + sb) < sAeNN[rta] * NC) ae[rta] = mfma64(a[sb][rta], b[sb][rta], ae[rta]);
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int row = 16 * rta + 4 * i + g;
-          if (row < NH && q < NRHS) {
-            double v;
-            if (q == 0) { v = A.Rh[e * NH + row] - ae[i]; A.fe[e * NH + row] = v; }
-            else { v = ae[i]; A.ae[e * (int64_t)NH * NH + row + NH * rhs_t<C>(q)] = v; }
-            if (!isfinite(v)) st |= 4;
+        for (int rta = 0; rta < RTH; ++rta)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int row = 16 * rta + 4 * i + g;
+            if (row < NH && q < NRHS) {
+              double v;
+              if (q == 0) { v = A.Rh[e * NH + row] - ae[rta][i]; A.fe[e * NH + row] = v; }
+              else { v = ae[rta][i]; A.ae[e * (int64_t)NH * NH + row + NH * rhs_t<C>(q)] = v; }
+              if (!isfinite(v)) st |= 4;
+            }
           }
-        }
       }
       __syncthreads();
     }
