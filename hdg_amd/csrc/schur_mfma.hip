@@ -33,8 +33,10 @@ struct MfmaShape {
   static constexpr int RT = (NU + 15) / 16;            // row tiles (= D' column tiles)
   static constexpr int R = 16 * RT;
   static constexpr int TRALL = (NRHS + 15) / 16;       // RHS tiles
-  // RHS tiles per pass: keep the accumulators at <= 96 doubles per lane
-  static constexpr int TRB0 = (96 / (4 * RT)) - RT;
+  // RHS tiles per pass: keep the accumulators at <= 96 (80 for 4 row tiles)
+  // doubles per lane
+  static constexpr int ACCB = RT >= 4 ? 80 : 96;
+  static constexpr int TRB0 = (ACCB / (4 * RT)) - RT;
   static constexpr int TRB = TRB0 < 1 ? 1 : (TRB0 > TRALL ? TRALL : TRB0);
   static constexpr int NPASS = (TRALL + TRB - 1) / TRB;
   static constexpr int CTP = RT + TRB;                 // tiles per pass
@@ -42,6 +44,9 @@ struct MfmaShape {
   static constexpr int RTH = (NH + 15) / 16;           // AE row tiles
   static constexpr int KB = C::NPV * C::NCU * C::ND;   // columns of B (q part of BD)
   static constexpr int KBS = (KB + 3) / 4;
+  // wavefronts per SIMD the register budget is set for: two (256 registers,
+  // accumulators in VGPRs) up to three row tiles, one beyond
+  static constexpr int WPE = RT <= 3 ? 2 : 1;
 };
 
 // state component of RHS column q >= 1 and its face-node index jf
@@ -74,7 +79,10 @@ HDGX_DEV constexpr bool bmie_needed(int q0, int s) {
 }
 
 template <class C>
-__global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+// two wavefronts per SIMD where the system fits: the compiler then keeps the
+// accumulator tiles in VGPRs within 256 registers (15% faster on NS p=3 than
+// one wavefront per SIMD with AGPR accumulators)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
   using S = MfmaShape<C>;
   constexpr int NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, NCU = C::NCU, ND = C::ND;
   constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NC = C::NC;
@@ -89,12 +97,15 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
   const int64_t e = blockIdx.x;
   if (e >= A.ne) return;
 
-  __shared__ double sPan[R * 4];
-  __shared__ double sZ[R * 4];
-  __shared__ double sRows[4 * 16 * CTP];
+  // strides padded so that lanes of a wavefront hit distinct LDS banks
+  constexpr int RP = R + 1;                       // sPan / sZ: column t at t*RP
+  constexpr int RSTR = 16 * CTP + 1;              // sRows: pivot row t at t*RSTR
+  constexpr int NCOLP = ((NCOL + 31) / 32) * 32 + 1;  // sXt: RHS column cc at cc*NCOLP
+  __shared__ double sPan[4 * RP];
+  __shared__ double sZ[4 * RP];
+  __shared__ double sRows[4 * RSTR];
   __shared__ int sStep[R];
-  __shared__ int sPr[4];
-  __shared__ double sXt[NCOL * 16];
+  __shared__ double sXt[NCOLP * 16];
   __shared__ double sR0[R];
   __shared__ double sMf[UQ ? NPV * NPV : 1];
   __shared__ double sMiRq[UQ ? NPV * NRQ : 1];
@@ -231,7 +242,7 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
     if constexpr (UQ) {
       // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns, A
       // fragments (columns of B) fetched a batch of k-steps ahead
-      constexpr int BB = 5;
+      constexpr int BB = RT >= 4 ? 2 : 4;
 #pragma unroll
       for (int s0 = 0; s0 < S::KBS; s0 += BB) {
         double a[BB][RT];
@@ -277,9 +288,14 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
     if (lane < R) sStep[lane] = -1;
     bool done = false;                        // lane r < R: row r already a pivot row
     __syncthreads();
+    // panels: unrolled over the panel's column tile (register indices are
+    // then static), a runtime loop over the four panels inside the tile
 #pragma unroll
-    for (int pn = 0; pn < S::NPAN; ++pn) {
-      const int k0 = 4 * pn, ct0 = k0 / 16, c0 = k0 % 16;
+    for (int ct0 = 0; ct0 < RT; ++ct0)
+#pragma unroll 1
+    for (int sub = 0; sub < 4; ++sub) {
+      const int c0 = 4 * sub, k0 = 16 * ct0 + c0;
+      if (k0 >= NU) break;
       // panel columns -> LDS, one row per lane
       {
         const int t = c - c0;
@@ -287,65 +303,84 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) sPan[(16 * rt + 4 * i + g) * 4 + t] = acc[rt][ct0][i];
+            for (int i = 0; i < 4; ++i) sPan[t * RP + 16 * rt + 4 * i + g] = acc[rt][ct0][i];
       }
       __syncthreads();
       double pr[4], z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-      for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[lane * 4 + t] : 0.0;
+      for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[t * RP + lane] : 0.0;
       int pvt[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        double v = -1.0;
-        if (lane < R && !done) { const double a = fabs(pr[t]); v = isnan(a) ? 1.0e308 : a; }
-        int p = lane;
-        wave_argmax(v, p);
-        if (!(v > 0.0)) st |= 2;
+        // |pr| as an ordered 64-bit key whose low 6 bits carry 63-lane, so one
+        // max-reduction finds the largest magnitude and, on ties, the lowest lane
+        unsigned long long key = 0ull;
+        if (lane < R && !done) {
+          const double a = fabs(pr[t]);
+          unsigned long long bits = isnan(a) ? 0x7ff0000000000000ull : (unsigned long long)__double_as_longlong(a);
+          key = (bits & ~63ull) | (unsigned long long)(63 - lane);
+        }
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          const unsigned long long o = __shfl_xor(key, off, 64);
+          key = o > key ? o : key;
+        }
+        if ((key >> 6) == 0ull) st |= 2;
+        const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63ull));
         pvt[t] = p;
         const double pv = __shfl(pr[t], p, 64);
+        const double ip = 1.0 / pv;
         z[t] = (lane == p) ? 1.0 : 0.0;
         double ppr[4], pz[4];
 #pragma unroll
         for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = __shfl(pr[t2], p, 64); pz[t2] = __shfl(z[t2], p, 64); }
         if (lane == p) {
-          const double ip = 1.0 / pv;
 #pragma unroll
           for (int t2 = 0; t2 < 4; ++t2) { pr[t2] *= ip; z[t2] *= ip; }
           done = true;
         } else {
-          const double m = pr[t] / pv;
+          const double m = pr[t] * ip;
 #pragma unroll
           for (int t2 = 0; t2 < 4; ++t2) { pr[t2] -= m * ppr[t2]; z[t2] -= m * pz[t2]; }
         }
-        if (lane == 0) { sStep[p] = k0 + t; sPr[t] = p; }
+        if (lane == 0) sStep[p] = k0 + t;
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= 1.0;
       if (lane < R)
 #pragma unroll
-        for (int t = 0; t < 4; ++t) sZ[lane * 4 + t] = z[t];
-      // original (pre-panel) pivot rows, columns from tile ct0 on
+        for (int t = 0; t < 4; ++t) sZ[t * RP + lane] = z[t];
+      // original (pre-panel) pivot rows, columns from tile ct0 on: the tile
+      // row and register holding pivot row p are uniform, so the selection is
+      // a scalar branch; lane group p&3 owns the row
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int p = pvt[t];
-        const int rtp = p >> 4, ip = (p & 15) >> 2, gp = p & 3;
+        const int sel = p >> 2;                     // 4*rt + i
+        double tmp[CTP];
+#pragma unroll
+        for (int ct = 0; ct < CTP; ++ct) tmp[ct] = 0.0;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
-            if (rt == rtp && i == ip && g == gp)
+            if (sel == 4 * rt + i)
 #pragma unroll
               for (int ct = 0; ct < CTP; ++ct)
-                if (ct >= ct0) sRows[t * 16 * CTP + 16 * ct + c] = acc[rt][ct][i];
+                if (ct >= ct0) tmp[ct] = acc[rt][ct][i];
+        if (g == (p & 3))
+#pragma unroll
+          for (int ct = 0; ct < CTP; ++ct)
+            if (ct >= ct0) sRows[t * RSTR + 16 * ct + c] = tmp[ct];
       }
       __syncthreads();
       // S += Z * S[pivot rows, :]
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
-        const double a = sZ[(16 * rt + c) * 4 + g];
+        const double a = sZ[g * RP + 16 * rt + c];
 #pragma unroll
         for (int ct = 0; ct < CTP; ++ct)
-          if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * 16 * CTP + 16 * ct + c], acc[rt][ct]);
+          if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
       }
       __syncthreads();
     }
@@ -360,7 +395,7 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = sStep[16 * rt + 4 * i + g];
-          if (k >= 0 && k < NU) sXt[k + NCOL * c] = acc[rt][RT + tb][i];
+          if (k >= 0 && k < NU) sXt[k + NCOLP * c] = acc[rt][RT + tb][i];
         }
       __syncthreads();
       if constexpr (UQ) {
@@ -370,10 +405,10 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
           const int n = rr % NPV, m = (rr / NPV) % NCU, dd = rr / (NPV * NCU);
           double v = 0.0;
 #pragma unroll
-          for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOL * c];
+          for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOLP * c];
           if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
           else if (q < NRHS && rhs_k<C>(q) == m) v -= sMiE[n + NPV * rhs_jf<C>(q) + NPV * NDF * dd];
-          sXt[NU + rr + NCOL * c] = v;
+          sXt[NU + rr + NCOLP * c] = v;
         }
         __syncthreads();
       }
@@ -382,7 +417,7 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
         const int q = q0 + cc;
         if (q >= NRHS) break;
         for (int r = lane; r < NCOL; r += 64) {
-          const double v = sXt[r + NCOL * cc];
+          const double v = sXt[r + NCOLP * cc];
           if (!isfinite(v)) st |= 4;
           if (q == 0) A.dudg[e * NCOL + r] = v;
           else Xe[r + (int64_t)NCOL * rhs_t<C>(q)] = v;
@@ -400,8 +435,11 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
             const int row = 16 * rta + 4 * i + g;
             ae[rta][i] = (row < NH && q >= 1 && q < NRHS) ? He[row + NH * rhs_t<C>(q)] : 0.0;
           }
-        constexpr int BATCH = 6;
+        constexpr int BATCH = RTH >= 4 ? 2 : 4;
+        int klen[RTH];
 #pragma unroll
+        for (int rta = 0; rta < RTH; ++rta) klen[rta] = __builtin_amdgcn_readfirstlane(sAeNN[rta] * NC);
+#pragma unroll 1
         for (int s0 = 0; s0 < KSMAX; s0 += BATCH) {
           double a[BATCH][RTH], b[BATCH][RTH];
 #pragma unroll
@@ -415,7 +453,7 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
                 const int row_a = 16 * rta + c;
                 if (K >= 0) {
                   if (row_a < NH) a[sb][rta] = GKe[row_a + NH * K];
-                  b[sb][rta] = sXt[K + NCOL * c];
+                  b[sb][rta] = sXt[K + NCOLP * c];
                 }
               }
             }
@@ -423,7 +461,7 @@ __global__ __launch_bounds__(64) void schur_mfma_kernel(SchurArgs A, const int32
           for (int sb = 0; sb < BATCH; ++sb)
 #pragma unroll
             for (int rta = 0; rta < RTH; ++rta)
-              if (s0 + sb < KSMAX && 4 * (s0 + sb) < sAeNN[rta] * NC) ae[rta] = mfma64(a[sb][rta], b[sb][rta], ae[rta]);
+              if (s0 + sb < KSMAX && 4 * (s0 + sb) < klen[rta]) ae[rta] = mfma64(a[sb][rta], b[sb][rta], ae[rta]);
         }
 #pragma unroll
         for (int rta = 0; rta < RTH; ++rta)
