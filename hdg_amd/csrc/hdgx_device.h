@@ -115,10 +115,12 @@ HDGX_DEV void ns2d_flux(double gam, double Re, double Pr, const TU* u, const TQ*
   f[3][1] = u[2] * h + uu * txy + vv * tyy + fc * Ty;
 }
 
-// Roe-form normal Jacobian (absflag 0) or |A_n| with the entropy fix of
-// parameter eps (absflag 1):  An = rl3*I + a*cc1^T + e*cc2^T.
+// Roe-form normal Jacobian applied to a vector, y = A_n(uh) x, without forming
+// the matrix: A_n = rl3*I + a*cc1^T + e*cc2^T with a = (1, vel, h),
+// e = (0, n, un) (ns/getan.m, fbou_euler3d.c); absflag 1 gives |A_n| with the
+// entropy fix of parameter eps.
 template <int ND, class T>
-HDGX_DEV void roe_an(double gam, double eps, int absflag, const T* uh, const double* n, T An[ND + 2][ND + 2]) {
+HDGX_DEV void roe_apply(double gam, double eps, int absflag, const T* uh, const double* n, const T* x, T* y) {
   const double gam1 = gam - 1.0;
   constexpr int NC = ND + 2;
   T r1 = 1.0 / uh[0];
@@ -146,24 +148,20 @@ HDGX_DEV void roe_an(double gam, double eps, int absflag, const T* uh, const dou
   T sm = s1 - rl3;
   T ic = 1.0 / c, ic2 = 1.0 / c2;
   T A1 = gam1 * sm * ic2, A2 = s2 * ic, B1 = gam1 * s2 * ic;
+  // d1 = cc1 . x, d2 = cc2 . x with b = (af, -vel, 1), d = (-un, n, 0):
+  // cc1 = A1*b + A2*d, cc2 = B1*b + sm*d
+  T d1 = (A1 * af - A2 * un) * x[0], d2 = (B1 * af - sm * un) * x[0];
 #pragma unroll
-  for (int k = 0; k < NC; ++k) {
-    // b = (af, -vel, 1), d = (-un, n, 0)
-    T bk = (k == 0) ? af : ((k <= ND) ? -vel[k - 1] : T(1.0));
-    T cc1, cc2;
-    if (k == 0) { cc1 = A1 * bk - A2 * un; cc2 = B1 * bk - sm * un; }
-    else if (k <= ND) { cc1 = A1 * bk + A2 * n[k - 1]; cc2 = B1 * bk + sm * n[k - 1]; }
-    else { cc1 = A1 * bk; cc2 = B1 * bk; }
-    // a = (1, vel, h), e = (0, n, un)
-    An[0][k] = (k == 0) ? cc1 + rl3 : cc1;
-#pragma unroll
-    for (int i = 0; i < ND; ++i) {
-      T v = cc1 * vel[i] + cc2 * n[i];
-      An[1 + i][k] = (k == 1 + i) ? v + rl3 : v;
-    }
-    T w = cc1 * h + cc2 * un;
-    An[ND + 1][k] = (k == ND + 1) ? w + rl3 : w;
+  for (int k = 1; k <= ND; ++k) {
+    d1 = d1 + (A2 * n[k - 1] - A1 * vel[k - 1]) * x[k];
+    d2 = d2 + (sm * n[k - 1] - B1 * vel[k - 1]) * x[k];
   }
+  d1 = d1 + A1 * x[NC - 1];
+  d2 = d2 + B1 * x[NC - 1];
+  y[0] = rl3 * x[0] + d1;
+#pragma unroll
+  for (int i = 0; i < ND; ++i) y[1 + i] = rl3 * x[1 + i] + vel[i] * d1 + n[i] * d2;
+  y[ND + 1] = rl3 * x[ND + 1] + h * d1 + un * d2;
 }
 
 // ----------------------------------------------------------- wave helpers

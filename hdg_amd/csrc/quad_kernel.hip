@@ -2,17 +2,26 @@
 // element computes everything volint + faceint produce (kernel/volint.m:1-263,
 // kernel/faceint.m:1-360) straight into the layouts the Schur step consumes.
 //
-// Dataflow per element:
-//   1. stage dgnodes, UDG, the gathered UH (and SH) of the element in LDS;
-//   2. one thread per volume Gauss point: geometry (volgeom), u at the point,
-//      flux + Jacobian, then the weighted rows wrk/wrl (volint.m:241-254);
-//      concurrently one thread per face Gauss point (another wavefront):
-//      facegeom, u and uh traces, fhat (all faces) and fbou (boundary faces),
+// Dataflow per element (each phase a flat task list over the workgroup):
+//   0. stage dgnodes, UDG, the gathered UH, SH and the small master tables in
+//      LDS;
+//   1. pointwise inputs: volume geometry (volgeom.m:22-59), face geometry
+//      (facegeom.m:13-52), and the state, trace and SH values at every
+//      Gauss point, one task per (point, component);
+//   2. physics, one task per (Gauss point, derivative direction): the flux
+//      (or fhat / fbou) is evaluated once in single-direction dual numbers,
+//      giving one column of its Jacobian; the results are the weighted rows
+//      wrk/wrl of volint.m:241-254 and the face rows of faceint.m:55-277
 //      pre-multiplied by the surface Jacobian;
-//   3. every output entry is a gather-sum over the staged rows, written once
-//      with consecutive threads on consecutive addresses.  The face scatter of
-//      faceint.m:322-360 (perm) becomes a per-node lookup loc(v, j) of node v
-//      on face j, so no entry is read-modify-written.
+//   3. outputs, each task owning a run of consecutive entries: BD one column
+//      (all npv test functions), F/GK/H/E/M/C/Rq/Ru/Rh single entries.  The
+//      face scatter of faceint.m:322-360 (perm) becomes the lookup loc(v, j)
+//      of volume node v on face j, so nothing is read-modify-written.
+//
+// The volume part of BD uses shapvgdotshapvl in its factored form
+// shapvg(jj,g,d) * shapvl(ii,g) (util/preprocess.m:23-37): a BD column
+// (c, ii, k) is sum_r shapvg(:, r) * (shapvl(ii, g_r) wrl(r, c, k)), whose
+// shapvg operand is uniform across the wavefront.
 #include <hip/hip_runtime.h>
 
 #include "hdgx_device.h"
@@ -21,179 +30,91 @@
 namespace hdgx {
 
 constexpr int QUAD_THREADS = 256;
-constexpr int QUAD_FACE_T0 = 128;   // face-point threads start in wavefront 2
 
-// f (NCU x ND) and f_udg (NCU x ND x NC) at one point (euler/flux.m, ns/flux2d.m,
-// laplacian AV f += eps_av * q).
-template <class C>
-HDGX_DEV void flux_point(const DevApp& ap, double av, const double* u, double f[C::NCU][C::ND],
-                         double fu[C::NCU][C::ND][C::NC]) {
-  constexpr int ND = C::ND, NCU = C::NCU, NC = C::NC;
+typedef Dual<1> D1;
+
+// Loads through the constant address space: with a wavefront-uniform index
+// they become scalar loads whose result feeds VALU instructions as an SGPR
+// operand (for master tables, which no kernel writes).
+typedef const __attribute__((address_space(4))) double* ConstD;
+typedef const __attribute__((address_space(4))) int32_t* ConstI;
+HDGX_DEV double uload(const double* p, int i) { return ((ConstD)p)[i]; }
+HDGX_DEV int32_t uload(const int32_t* p, int i) { return ((ConstI)p)[i]; }
+
+// f (NCU x ND) at one point (euler/flux.m, ns/flux2d.m; laplacian AV f += av*q
+// as in nsshock/flux.m:16-21)
+template <class C, class T>
+HDGX_DEV void vol_flux(const DevApp& ap, double av, const T* u, T f[C::NCU][C::ND]) {
   if constexpr (C::PHYS == PHYS_EULER) {
-    Dual<NCU> w[NCU];
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) w[c] = Dual<NCU>::seed(u[c], c);
-    Dual<NCU> fl[NCU][ND];
-    euler_flux<ND>(ap.gam, w, fl);
-#pragma unroll
-    for (int c = 0; c < NCU; ++c)
-#pragma unroll
-      for (int d = 0; d < ND; ++d) {
-        f[c][d] = fl[c][d].v;
-#pragma unroll
-        for (int k = 0; k < NC; ++k) fu[c][d][k] = fl[c][d].d[k];
-      }
+    euler_flux<C::ND>(ap.gam, u, f);
   } else {
-    // pass 1: derivatives w.r.t. u (q held fixed)
-    {
-      Dual<4> w[4];
+    ns2d_flux(ap.gam, ap.Re, ap.Pr, u, u + 4, f);
 #pragma unroll
-      for (int c = 0; c < 4; ++c) w[c] = Dual<4>::seed(u[c], c);
-      Dual<4> fl[4][2];
-      ns2d_flux(ap.gam, ap.Re, ap.Pr, w, u + 4, fl);
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int d = 0; d < 2; ++d) {
-          f[c][d] = fl[c][d].v + av * u[4 + c + 4 * d];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) fu[c][d][k] = fl[c][d].d[k];
-        }
-    }
-    // pass 2: derivatives w.r.t. q (linear in q)
-    {
-      Dual<8> w[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) w[c] = Dual<8>::seed(u[4 + c], c);
-      Dual<8> fl[4][2];
-      ns2d_flux(ap.gam, ap.Re, ap.Pr, u, w, fl);
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-#pragma unroll
-        for (int d = 0; d < 2; ++d)
-#pragma unroll
-          for (int k = 0; k < 8; ++k) fu[c][d][4 + k] = fl[c][d].d[k] + ((k == c + 4 * d) ? av : 0.0);
-    }
+      for (int d = 0; d < 2; ++d) f[c][d] = f[c][d] + av * u[4 + c + 4 * d];
   }
 }
 
-// fhat = f(uh[,q]).n + tau (u - uh) (euler/fhat.m caseFlux 1/caseStab 4, ns/fhat.m:49-67)
-template <class C>
-HDGX_DEV void fhat_point(const DevApp& ap, double av, const double* n, const double* u, const double* uh,
-                         double fh[C::NCU], double fhu[C::NCU][C::NC], double fhh[C::NCU][C::NCU]) {
-  constexpr int ND = C::ND, NCU = C::NCU, NC = C::NC;
-  const double tau = ap.tau;
+// fhat = f(uh[, q]).n + tau (u - uh) (euler/fhat.m caseFlux 1 / caseStab 4,
+// ns/fhat.m:49-67; AV adds av q.n)
+template <class C, class T>
+HDGX_DEV void face_fhat(const DevApp& ap, double av, const double* n, const T* u, const T* uh, T fh[C::NCU]) {
+  constexpr int ND = C::ND, NCU = C::NCU;
+  T fl[NCU][ND];
+  if constexpr (C::PHYS == PHYS_EULER) euler_flux<ND>(ap.gam, uh, fl);
+  else ns2d_flux(ap.gam, ap.Re, ap.Pr, uh, u + 4, fl);
 #pragma unroll
-  for (int c = 0; c < NCU; ++c)
+  for (int c = 0; c < NCU; ++c) {
+    T s = fl[c][0] * n[0];
 #pragma unroll
-    for (int k = 0; k < NC; ++k) fhu[c][k] = (k == c) ? tau : 0.0;
-  if constexpr (C::PHYS == PHYS_EULER) {
-    Dual<NCU> w[NCU];
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) w[c] = Dual<NCU>::seed(uh[c], c);
-    Dual<NCU> fl[NCU][ND];
-    euler_flux<ND>(ap.gam, w, fl);
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) {
-      Dual<NCU> fn(0.0);
-#pragma unroll
-      for (int d = 0; d < ND; ++d) fn = fn + fl[c][d] * n[d];
-      fh[c] = fn.v + tau * (u[c] - uh[c]);
-#pragma unroll
-      for (int k = 0; k < NCU; ++k) fhh[c][k] = fn.d[k] - ((c == k) ? tau : 0.0);
-    }
-  } else {
-    {
-      Dual<4> w[4];
-#pragma unroll
-      for (int c = 0; c < 4; ++c) w[c] = Dual<4>::seed(uh[c], c);
-      Dual<4> fl[4][2];
-      ns2d_flux(ap.gam, ap.Re, ap.Pr, w, u + 4, fl);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        Dual<4> fn = fl[c][0] * n[0] + fl[c][1] * n[1];
-        fh[c] = fn.v + av * (u[4 + c] * n[0] + u[8 + c] * n[1]) + tau * (u[c] - uh[c]);
-#pragma unroll
-        for (int k = 0; k < 4; ++k) fhh[c][k] = fn.d[k] - ((c == k) ? tau : 0.0);
-      }
-    }
-    {
-      Dual<8> w[8];
-#pragma unroll
-      for (int c = 0; c < 8; ++c) w[c] = Dual<8>::seed(u[4 + c], c);
-      Dual<8> fl[4][2];
-      ns2d_flux(ap.gam, ap.Re, ap.Pr, uh, w, fl);
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        Dual<8> fn = fl[c][0] * n[0] + fl[c][1] * n[1];
-#pragma unroll
-        for (int k = 0; k < 8; ++k)
-          fhu[c][4 + k] = fn.d[k] + ((k == c) ? av * n[0] : 0.0) + ((k == c + 4) ? av * n[1] : 0.0);
-      }
-    }
+    for (int d = 1; d < ND; ++d) s = s + fl[c][d] * n[d];
+    if constexpr (C::PHYS == PHYS_NS) s = s + av * (u[4 + c] * n[0] + u[8 + c] * n[1]);
+    fh[c] = s + ap.tau * (u[c] - uh[c]);
   }
 }
 
-// Boundary flux: far field (ib 1: euler/fbou.m:24-29 LLF in 2D Euler, Roe form
-// for NS and 3D Euler: ns/fbou.m:37-46, fbou_euler3d.c:20-60) and the 2D Euler
-// inviscid wall (ib 2: euler/fbou.m:30-48).  Host validation rejects others.
-template <class C>
-HDGX_DEV void fbou_point(const DevApp& ap, int ib, const double* uinf, const double* n, const double* u,
-                         const double* uh, double fh[C::NCU], double fhu[C::NCU][C::NC],
-                         double fhh[C::NCU][C::NCU]) {
-  constexpr int ND = C::ND, NCU = C::NCU, NC = C::NC;
-#pragma unroll
-  for (int c = 0; c < NCU; ++c)
-#pragma unroll
-    for (int k = 0; k < NC; ++k) fhu[c][k] = 0.0;
+// Boundary flux: far field (ib 1) fh = (an + An)(u - uh) - (an - An)(uinf - uh)
+// = an (u - uinf) + An (u + uinf - 2 uh) with the LLF pair an = |un| + c,
+// An = un - c in 2D Euler (euler/fbou.m:24-29, getan.m) and the Roe pair
+// an = A_n, An = |A_n| for NS and 3D Euler (ns/fbou.m:37-46,
+// fbou_euler3d.c:20-60); 2D Euler inviscid wall (ib 2: euler/fbou.m:30-48).
+// Host validation rejects other boundary types.
+template <class C, class T>
+HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const double* n, const T* u, const T* uh,
+                        T fh[C::NCU]) {
+  constexpr int ND = C::ND, NCU = C::NCU;
   if (ib == 2) {
     if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
-      double un = u[1] * n[0] + u[2] * n[1];
-      double ub[4] = {u[0], u[1] - n[0] * un, u[2] - n[1] * un, u[3]};
-#pragma unroll
-      for (int c = 0; c < 4; ++c) {
-        fh[c] = ub[c] - uh[c];
-#pragma unroll
-        for (int k = 0; k < 4; ++k) fhh[c][k] = (c == k) ? -1.0 : 0.0;
-      }
-      fhu[0][0] = 1.0;
-      fhu[1][1] = 1.0 - n[0] * n[0];
-      fhu[1][2] = -n[0] * n[1];
-      fhu[2][1] = -n[0] * n[1];
-      fhu[2][2] = 1.0 - n[1] * n[1];
-      fhu[3][3] = 1.0;
+      T un = u[1] * n[0] + u[2] * n[1];
+      fh[0] = u[0] - uh[0];
+      fh[1] = (u[1] - n[0] * un) - uh[1];
+      fh[2] = (u[2] - n[1] * un) - uh[2];
+      fh[3] = u[3] - uh[3];
     }
     return;
   }
-  Dual<NCU> w[NCU];
-#pragma unroll
-  for (int c = 0; c < NCU; ++c) w[c] = Dual<NCU>::seed(uh[c], c);
-  Dual<NCU> an[NCU][NCU], An[NCU][NCU];
+  T x[NCU], y[NCU];
   if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
     const double gam1 = ap.gam - 1.0;
-    Dual<NCU> r1 = 1.0 / w[0];
-    Dual<NCU> uv = w[1] * r1, vv = w[2] * r1;
-    Dual<NCU> af = 0.5 * (uv * uv + vv * vv);
-    Dual<NCU> p = gam1 * (w[3] - w[0] * af);
-    Dual<NCU> c = dsqrt(ap.gam * p * r1);
-    Dual<NCU> un = uv * n[0] + vv * n[1];
-    Dual<NCU> l2 = dabs(un) + c, l3 = un - c;
+    T r1 = 1.0 / uh[0];
+    T uv = uh[1] * r1, vv = uh[2] * r1;
+    T af = 0.5 * (uv * uv + vv * vv);
+    T p = gam1 * (uh[3] - uh[0] * af);
+    T c = dsqrt(ap.gam * p * r1);
+    T un = uv * n[0] + vv * n[1];
+    T l2 = dabs(un) + c, l3 = un - c;
 #pragma unroll
-    for (int i = 0; i < NCU; ++i)
-#pragma unroll
-      for (int k = 0; k < NCU; ++k) { an[i][k] = (i == k) ? l2 : Dual<NCU>(0.0); An[i][k] = (i == k) ? l3 : Dual<NCU>(0.0); }
+    for (int k = 0; k < NCU; ++k) fh[k] = l2 * (u[k] - uinf[k]) + l3 * ((u[k] - uh[k]) + (uinf[k] - uh[k]));
   } else {
-    roe_an<ND>(ap.gam, ap.eps, 0, w, n, an);
-    roe_an<ND>(ap.gam, ap.eps, 1, w, n, An);
-  }
 #pragma unroll
-  for (int i = 0; i < NCU; ++i) {
-    Dual<NCU> s(0.0);
+    for (int k = 0; k < NCU; ++k) x[k] = u[k] - uinf[k];
+    roe_apply<ND>(ap.gam, ap.eps, 0, uh, n, x, y);
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) s = s + (an[i][k] + An[i][k]) * (u[k] - w[k]) - (an[i][k] - An[i][k]) * (uinf[k] - w[k]);
-    fh[i] = s.v;
+    for (int k = 0; k < NCU; ++k) { fh[k] = y[k]; x[k] = (u[k] - uh[k]) + (uinf[k] - uh[k]); }
+    roe_apply<ND>(ap.gam, ap.eps, 1, uh, n, x, y);
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) { fhu[i][k] = an[i][k].v + An[i][k].v; fhh[i][k] = s.d[k]; }
+    for (int k = 0; k < NCU; ++k) fh[k] = fh[k] + y[k];
   }
 }
 
@@ -202,195 +123,281 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
   constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
+  constexpr int NDIRF = NC + NCU;        // face derivative directions: UDG components, then UH
   constexpr bool UQ = C::UQ;
-  static_assert(NGV <= QUAD_FACE_T0 && NFG <= NT - QUAD_FACE_T0, "too many quadrature points");
 
   const int tid = threadIdx.x;
   const int64_t e = blockIdx.x;
   if (e >= A.ne) return;
   const int ncd = ap.ncd;
+  const bool tdep = ap.tdep != 0;
 
   __shared__ double sX[NPV * (ND + 1)];
   __shared__ double sU[NPV * NC];
-  __shared__ double sUH[NCU * NDF];
+  __shared__ double sUH[NH];
   __shared__ double sSH[NPV * NC];
-  __shared__ double sJac[NGV];
-  __shared__ double sXx[NGV * ND * ND];
+  __shared__ int32_t sBf[NFE];
+  __shared__ int32_t sLoc[NPV * NFE];
+  __shared__ double sShV[NGV * NPV];          // shapvt(:, :, 1)
+  __shared__ double sShF[NGF * NPF];          // shapft(:, :, 1)
+  __shared__ double sSff[NPF * NPF * NGF];    // shapfgdotshapfc(:, :, 1)
+  __shared__ double sJac[NGV], sXx[NGV * ND * ND], sAv[NGV];
+  __shared__ double sUg[NGV * NC], sSg[NGV * NCU];
+  __shared__ double sNl[NFG * ND], sFJ[NFG], sAvf[NFG];
+  __shared__ double sUf[NFG * NC], sUHf[NFG * NCU];
   __shared__ double sWrk[NR * NCU];
   __shared__ double sWrl[NR * NCU * NC];
-  __shared__ double sNJ[NFG * ND];
   __shared__ double sFH[NFG * NCU], sFHu[NFG * NCU * NC], sFHh[NFG * NCU * NCU];
   __shared__ double sGH[NFG * NCU], sGHu[NFG * NCU * NC], sGHh[NFG * NCU * NCU];
   __shared__ double sM[UQ ? NPV * NPV : 1], sC[UQ ? NPV * NPV * ND : 1], sE[UQ ? NPV * NDF * ND : 1];
-  __shared__ int32_t sBf[NFE];
 
-  // ---- 1. stage the element's inputs
+  // ---- 0. stage the element's inputs and the small master tables
   for (int i = tid; i < NPV * ncd; i += NT) sX[i] = A.dgnodes[e * NPV * ncd + i];
   for (int i = tid; i < NPV * NC; i += NT) sU[i] = A.udg[e * NPV * NC + i];
-  for (int i = tid; i < NCU * NDF; i += NT) sUH[i] = A.uh[e * NH + i];
-  if (UQ || ap.tdep)
+  for (int i = tid; i < NH; i += NT) sUH[i] = A.uh[e * NH + i];
+  if (UQ || tdep)
     for (int i = tid; i < NPV * NC; i += NT) sSH[i] = A.sh ? A.sh[e * NPV * NC + i] : 0.0;
   if (tid < NFE) sBf[tid] = A.bf[e * NFE + tid];
+  for (int i = tid; i < NPV * NFE; i += NT) sLoc[i] = T.loc[i];
+  for (int i = tid; i < NGV * NPV; i += NT) sShV[i] = T.shapvt[i];
+  for (int i = tid; i < NGF * NPF; i += NT) sShF[i] = T.shapft[i];
+  for (int i = tid; i < NPF * NPF * NGF; i += NT) sSff[i] = T.sff[i];
   __syncthreads();
 
-  // ---- 2a. volume Gauss points (volgeom.m:22-59, volint.m:30-254)
-  if (tid < NGV) {
-    const int g = tid;
-    double J[ND][ND];
+  // ---- 1. pointwise inputs
+  {
+    constexpr int W0 = NGV, W1 = W0 + NFG, W2 = W1 + NGV * NC, W3 = W2 + NFG * NC, W4 = W3 + NFG * NCU;
+    const int W5 = W4 + (tdep ? NGV * NCU : 0);
+    for (int w = tid; w < W5; w += NT) {
+      if (w < W0) {
+        // volume geometry: J, jac = det J, Xx(a, b) = jac * dxi_b/dx_a
+        const int g = w;
+        double J[ND][ND];
 #pragma unroll
-    for (int j = 0; j < ND; ++j)
+        for (int j = 0; j < ND; ++j)
 #pragma unroll
-      for (int i = 0; i < ND; ++i) {
+          for (int i = 0; i < ND; ++i) {
+            double v = 0.0;
+#pragma unroll
+            for (int a = 0; a < NPV; ++a) v += T.shapmv[g + NGV * a + NGV * NPV * (1 + j)] * sX[a + NPV * i];
+            J[j][i] = v;   // dx_i / dxi_j
+          }
+        double jac, Xx[ND][ND];
+        if constexpr (ND == 2) {
+          jac = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+          Xx[0][0] = J[1][1]; Xx[1][0] = -J[1][0]; Xx[0][1] = -J[0][1]; Xx[1][1] = J[0][0];
+        } else {
+          jac = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[2][1] * J[1][2] + J[1][0] * J[2][1] * J[0][2] -
+                J[1][0] * J[0][1] * J[2][2] + J[2][0] * J[0][1] * J[1][2] - J[2][0] * J[1][1] * J[0][2];
+          Xx[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+          Xx[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+          Xx[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+          Xx[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+          Xx[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+          Xx[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+          Xx[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+          Xx[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+          Xx[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        }
+        sJac[g] = jac;
+#pragma unroll
+        for (int a = 0; a < ND; ++a)
+#pragma unroll
+          for (int b = 0; b < ND; ++b) sXx[g * ND * ND + a * ND + b] = Xx[a][b];
+        double av = 0.0;
+        if (ap.av_col >= 0)
+          for (int a = 0; a < NPV; ++a) av += T.shapmv[g + NGV * a] * sX[a + NPV * ap.av_col];
+        sAv[g] = av;
+      } else if (w < W1) {
+        // face geometry: unit normal and surface Jacobian
+        const int t = w - W0;
+        const int j = t / NGF, g = t - j * NGF;
+        double dpg[ND][ND > 1 ? ND - 1 : 1];
+#pragma unroll
+        for (int d = 0; d < ND; ++d)
+#pragma unroll
+          for (int k = 0; k < ND - 1; ++k) {
+            double v = 0.0;
+#pragma unroll
+            for (int a = 0; a < NPF; ++a)
+              v += T.shapmf[g + NGF * a + NGF * NPF * (1 + k)] * sX[T.permgeom[a + NPF * j] + NPV * d];
+            dpg[d][k] = v;
+          }
+        double nl[ND], jac;
+        if constexpr (ND == 2) {
+          jac = sqrt(dpg[0][0] * dpg[0][0] + dpg[1][0] * dpg[1][0]);
+          nl[0] = dpg[1][0] / jac;
+          nl[1] = -dpg[0][0] / jac;
+        } else {
+          double n0 = dpg[1][0] * dpg[2][1] - dpg[2][0] * dpg[1][1];
+          double n1 = dpg[2][0] * dpg[0][1] - dpg[0][0] * dpg[2][1];
+          double n2 = dpg[0][0] * dpg[1][1] - dpg[1][0] * dpg[0][1];
+          jac = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+          nl[0] = n0 / jac; nl[1] = n1 / jac; nl[2] = n2 / jac;
+        }
+#pragma unroll
+        for (int i = 0; i < ND; ++i) sNl[t + NFG * i] = nl[i];
+        sFJ[t] = jac;
+        double av = 0.0;
+        if (ap.av_col >= 0)
+          for (int a = 0; a < NPF; ++a) av += T.shapmf[g + NGF * a] * sX[T.permgeom[a + NPF * j] + NPV * ap.av_col];
+        sAvf[t] = av;
+      } else if (w < W2) {
+        const int i = w - W1, g = i % NGV, c = i / NGV;
         double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPV; ++a) v += T.shapmv[g + NGV * a + NGV * NPV * (1 + j)] * sX[a + NPV * i];
-        J[j][i] = v;   // dx_i / dxi_j
+        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sU[a + NPV * c];
+        sUg[i] = v;
+      } else if (w < W3) {
+        const int i = w - W2, t = i % NFG, c = i / NFG;
+        const int j = t / NGF, g = t - j * NGF;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sU[T.perm[a + NPF * j] + NPV * c];
+        sUf[i] = v;
+      } else if (w < W4) {
+        const int i = w - W3, t = i % NFG, c = i / NFG;
+        const int j = t / NGF, g = t - j * NGF;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sUH[c + NCU * (a + NPF * j)];
+        sUHf[i] = v;
+      } else {
+        const int i = w - W4, g = i % NGV, c = i / NGV;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sSH[a + NPV * c];
+        sSg[i] = v;
       }
-    double jac, Xx[ND][ND];   // Xx[a][b] = jac * dxi_b / dx_a
-    if constexpr (ND == 2) {
-      jac = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-      Xx[0][0] = J[1][1]; Xx[1][0] = -J[1][0]; Xx[0][1] = -J[0][1]; Xx[1][1] = J[0][0];
-    } else {
-      jac = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[2][1] * J[1][2] + J[1][0] * J[2][1] * J[0][2] -
-            J[1][0] * J[0][1] * J[2][2] + J[2][0] * J[0][1] * J[1][2] - J[2][0] * J[1][1] * J[0][2];
-      Xx[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-      Xx[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-      Xx[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-      Xx[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-      Xx[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-      Xx[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-      Xx[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-      Xx[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-      Xx[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
     }
-    sJac[g] = jac;
+  }
+  __syncthreads();
+
+  // ---- 2. physics, one task per (Gauss point, derivative direction)
+  for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
+    if (w < NGV * NC) {
+      // volume point g, direction UDG component k (volint.m:30-254)
+      const int g = w % NGV, k = w / NGV;
+      D1 u[NC];
 #pragma unroll
-    for (int a = 0; a < ND; ++a)
+      for (int c = 0; c < NC; ++c) { u[c].v = sUg[g + NGV * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
+      D1 f[NCU][ND];
+      vol_flux<C>(ap, sAv[g], u, f);
+      const double jac = sJac[g];
+      double Xx[ND][ND];
 #pragma unroll
-      for (int b = 0; b < ND; ++b) sXx[g * ND * ND + a * ND + b] = Xx[a][b];
-    double av = 0.0;
-    if (ap.av_col >= 0) {
-      for (int a = 0; a < NPV; ++a) av += T.shapmv[g + NGV * a] * sX[a + NPV * ap.av_col];
-    }
-    double ug[NC];
+      for (int a = 0; a < ND; ++a)
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      double v = 0.0;
-#pragma unroll
-      for (int a = 0; a < NPV; ++a) v += T.shapvt[g + NGV * a] * sU[a + NPV * c];
-      ug[c] = v;
-    }
-    double f[NCU][ND], fu[NCU][ND][NC];
-    flux_point<C>(ap, av, ug, f, fu);
-    // zero source (euler/source.m, ns/source.m) plus the BDF time term
-    for (int c = 0; c < NCU; ++c) {
-      double s = 0.0;
-      if (ap.tdep) {
-        double st = 0.0;
-        for (int a = 0; a < NPV; ++a) st += T.shapvt[g + NGV * a] * sSH[a + NPV * c];
-        s = st - ug[c] * ap.fc_u;
-      }
-      sWrk[g + NR * c] = s * jac;
-      for (int k = 0; k < NC; ++k) sWrl[g + NR * (c + NCU * k)] = (ap.tdep && k == c) ? ap.fc_u * jac : 0.0;
-    }
-#pragma unroll
-    for (int i = 0; i < ND; ++i)
+        for (int b = 0; b < ND; ++b) Xx[a][b] = sXx[g * ND * ND + a * ND + b];
 #pragma unroll
       for (int c = 0; c < NCU; ++c) {
-        double v = 0.0;
+        sWrl[g + NR * (c + NCU * k)] = (tdep && k == c) ? ap.fc_u * jac : 0.0;
 #pragma unroll
-        for (int j = 0; j < ND; ++j) v += f[c][j] * Xx[j][i];
-        sWrk[g + NGV * (i + 1) + NR * c] = v;
+        for (int i = 0; i < ND; ++i) {
+          double v = 0.0;
 #pragma unroll
-        for (int k = 0; k < NC; ++k) {
-          double w = 0.0;
-#pragma unroll
-          for (int j = 0; j < ND; ++j) w += fu[c][j][k] * Xx[j][i];
-          sWrl[g + NGV * (i + 1) + NR * (c + NCU * k)] = -w;
+          for (int j = 0; j < ND; ++j) v += f[c][j].d[0] * Xx[j][i];
+          sWrl[g + NGV * (i + 1) + NR * (c + NCU * k)] = -v;
         }
       }
-  }
-
-  // ---- 2b. face Gauss points (facegeom.m:13-52, faceint.m:55-277)
-  if (tid >= QUAD_FACE_T0 && tid < QUAD_FACE_T0 + NFG) {
-    const int t = tid - QUAD_FACE_T0;
-    const int j = t / NGF, g = t - j * NGF;
-    double dpg[ND][ND > 1 ? ND - 1 : 1];
+      if (k == 0) {
+        // zero source (euler/source.m, ns/source.m) plus the BDF time term
 #pragma unroll
-    for (int d = 0; d < ND; ++d)
+        for (int c = 0; c < NCU; ++c) {
+          const double s = tdep ? sSg[g + NGV * c] - u[c].v * ap.fc_u : 0.0;
+          sWrk[g + NR * c] = s * jac;
 #pragma unroll
-      for (int k = 0; k < ND - 1; ++k) {
-        double v = 0.0;
+          for (int i = 0; i < ND; ++i) {
+            double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPF; ++a) v += T.shapmf[g + NGF * a + NGF * NPF * (1 + k)] * sX[T.permgeom[a + NPF * j] + NPV * d];
-        dpg[d][k] = v;
+            for (int j = 0; j < ND; ++j) v += f[c][j].v * Xx[j][i];
+            sWrk[g + NGV * (i + 1) + NR * c] = v;
+          }
+        }
       }
-    double nl[ND], jac;
-    if constexpr (ND == 2) {
-      jac = sqrt(dpg[0][0] * dpg[0][0] + dpg[1][0] * dpg[1][0]);
-      nl[0] = dpg[1][0] / jac;
-      nl[1] = -dpg[0][0] / jac;
     } else {
-      double n0 = dpg[1][0] * dpg[2][1] - dpg[2][0] * dpg[1][1];
-      double n1 = dpg[2][0] * dpg[0][1] - dpg[0][0] * dpg[2][1];
-      double n2 = dpg[0][0] * dpg[1][1] - dpg[1][0] * dpg[0][1];
-      jac = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-      nl[0] = n0 / jac; nl[1] = n1 / jac; nl[2] = n2 / jac;
-    }
+      // face point t, direction k: UDG component k (k < NC) or UH component
+      // k - NC (faceint.m:55-277)
+      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+      const int j = t / NGF;
+      D1 u[NC], uh[NCU];
 #pragma unroll
-    for (int i = 0; i < ND; ++i) sNJ[t + NFG * i] = nl[i] * jac;
-    double av = 0.0;
-    if (ap.av_col >= 0)
-      for (int a = 0; a < NPF; ++a) av += T.shapmf[g + NGF * a] * sX[T.permgeom[a + NPF * j] + NPV * ap.av_col];
-    double ug[NC], uhg[NCU];
+      for (int c = 0; c < NC; ++c) { u[c].v = sUf[t + NFG * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
 #pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      double v = 0.0;
+      for (int c = 0; c < NCU; ++c) { uh[c].v = sUHf[t + NFG * c]; uh[c].d[0] = (c + NC == k) ? 1.0 : 0.0; }
+      double nl[ND];
 #pragma unroll
-      for (int a = 0; a < NPF; ++a) v += T.shapft[g + NGF * a] * sU[T.perm[a + NPF * j] + NPV * c];
-      ug[c] = v;
-    }
+      for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+      const double jac = sFJ[t];
+      D1 fh[NCU];
+      face_fhat<C>(ap, sAvf[t], nl, u, uh, fh);
 #pragma unroll
-    for (int c = 0; c < NCU; ++c) {
-      double v = 0.0;
+      for (int c = 0; c < NCU; ++c) {
+        if (k < NC) sFHu[t + NFG * (c + NCU * k)] = fh[c].d[0] * jac;
+        else sFHh[t + NFG * (c + NCU * (k - NC))] = fh[c].d[0] * jac;
+        if (k == 0) sFH[t + NFG * c] = fh[c].v * jac;
+      }
+      const int bfj = sBf[j];
+      if (bfj < 0) {
+        const int id = -bfj - 1;
+        double uinf[NCU];
 #pragma unroll
-      for (int a = 0; a < NPF; ++a) v += T.shapft[g + NGF * a] * sUH[c + NCU * (a + NPF * j)];
-      uhg[c] = v;
-    }
-    double fh[NCU], fhu[NCU][NC], fhh[NCU][NCU];
-    fhat_point<C>(ap, av, nl, ug, uhg, fh, fhu, fhh);
+        for (int c = 0; c < NCU; ++c) uinf[c] = ap.bcs[id * 5 + c];
+        face_fbou<C>(ap, ap.bcm[id], uinf, nl, u, uh, fh);
+      }
 #pragma unroll
-    for (int c = 0; c < NCU; ++c) {
-      sFH[t + NFG * c] = fh[c] * jac;
-#pragma unroll
-      for (int k = 0; k < NC; ++k) sFHu[t + NFG * (c + NCU * k)] = fhu[c][k] * jac;
-#pragma unroll
-      for (int k = 0; k < NCU; ++k) sFHh[t + NFG * (c + NCU * k)] = fhh[c][k] * jac;
-    }
-    const int bfj = sBf[j];
-    if (bfj < 0) {
-      const int id = -bfj - 1;
-      const int ib = ap.bcm[id];
-      double uinf[NCU];
-#pragma unroll
-      for (int c = 0; c < NCU; ++c) uinf[c] = ap.bcs[id * 5 + c];
-      fbou_point<C>(ap, ib, uinf, nl, ug, uhg, fh, fhu, fhh);
-    }
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) {
-      sGH[t + NFG * c] = fh[c] * jac;
-#pragma unroll
-      for (int k = 0; k < NC; ++k) sGHu[t + NFG * (c + NCU * k)] = fhu[c][k] * jac;
-#pragma unroll
-      for (int k = 0; k < NCU; ++k) sGHh[t + NFG * (c + NCU * k)] = fhh[c][k] * jac;
+      for (int c = 0; c < NCU; ++c) {
+        if (k < NC) sGHu[t + NFG * (c + NCU * k)] = fh[c].d[0] * jac;
+        else sGHh[t + NFG * (c + NCU * (k - NC))] = fh[c].d[0] * jac;
+        if (k == 0) sGH[t + NFG * c] = fh[c].v * jac;
+      }
     }
   }
   __syncthreads();
 
-  const int32_t* loc = T.loc;   // loc[v + NPV*j]
-  const double* sff = T.sff;    // sff[a + NPF*b + NPF*NPF*g]
+  // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
+  // faceint.m:316,326), one column (c, ii, k) per task
+  double* BD = A.BD + e * (int64_t)NU * C::NCOL;
+  for (int col = tid; col < NCU * NPV * NC; col += NT) {
+    const int c = col % NCU, ii = (col / NCU) % NPV, k = col / (NCU * NPV);
+    const double* wl = sWrl + NR * (c + NCU * k);
+    double acc[NPV];
+#pragma unroll
+    for (int jj = 0; jj < NPV; ++jj) acc[jj] = 0.0;
+    double sv[NGV];
+#pragma unroll
+    for (int g = 0; g < NGV; ++g) sv[g] = sShV[g + NGV * ii];
+#pragma unroll
+    for (int d = 0; d <= ND; ++d)
+#pragma unroll
+      for (int g = 0; g < NGV; ++g) {
+        const int r = g + NGV * d;
+        const double tv = sv[g] * wl[r];
+#pragma unroll
+        for (int jj = 0; jj < NPV; ++jj) acc[jj] += uload(T.shapvg, jj + NPV * r) * tv;
+      }
+#pragma unroll
+    for (int j = 0; j < NFE; ++j) {
+      const int lb = sLoc[ii + NPV * j];
+      if (lb >= 0) {
+        const double* fu = sFHu + j * NGF + NFG * (c + NCU * k);
+#pragma unroll
+        for (int jj = 0; jj < NPV; ++jj) {
+          const int la = uload(T.loc, jj + NPV * j);
+          if (la >= 0) {
+            double v = 0.0;
+#pragma unroll
+            for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * lb + NPF * NPF * g] * fu[g];
+            acc[jj] += v;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int jj = 0; jj < NPV; ++jj) BD[jj + NPV * col] = acc[jj];
+  }
 
-  // ---- 3a. Ru = shapvg*wrk - sum_faces shapfg*FH (volint.m:258, faceint.m:302-330)
+  // ---- 3b. Ru = shapvg*wrk - sum_faces shapfg*FH (volint.m:258, faceint.m:302-330)
   double* Ru = A.Ru + e * NU;
   for (int idx = tid; idx < NU; idx += NT) {
     const int a = idx % NPV, c = idx / NPV;
@@ -398,41 +405,19 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
     for (int r = 0; r < NR; ++r) v += T.shapvg[a + NPV * r] * sWrk[r + NR * c];
 #pragma unroll
     for (int j = 0; j < NFE; ++j) {
-      const int la = loc[a + NPV * j];
+      const int la = sLoc[a + NPV * j];
       if (la >= 0)
         for (int g = 0; g < NGF; ++g) v -= T.shapfg[la + NPF * g] * sFH[j * NGF + g + NFG * c];
     }
     Ru[idx] = v;
   }
-  // ---- 3b. Rh = -shapfg * (fbou | fhat) (faceint.m:335-339)
+  // ---- 3c. Rh = -shapfg * (fbou | fhat) (faceint.m:335-339)
   double* Rh = A.Rh + e * NH;
   for (int idx = tid; idx < NH; idx += NT) {
     const int c = idx % NCU, ab = idx / NCU, j = ab / NPF, a = ab - j * NPF;
     double v = 0.0;
     for (int g = 0; g < NGF; ++g) v += T.shapfg[a + NPF * g] * sGH[j * NGF + g + NFG * c];
     Rh[idx] = -v;
-  }
-  // ---- 3c. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262, faceint.m:316,326)
-  double* BD = A.BD + e * (int64_t)NU * C::NCOL;
-  for (int idx = tid; idx < NU * C::NCOL; idx += NT) {
-    const int jj = idx % NPV;
-    int rest = idx / NPV;
-    const int c = rest % NCU;
-    rest /= NCU;
-    const int ii = rest % NPV;
-    const int k = rest / NPV;
-    const double* sv = T.svv + jj + NPV * ii;
-    const double* wl = sWrl + NR * (c + NCU * k);
-    double v = 0.0;
-    for (int r = 0; r < NR; ++r) v += sv[NPV * NPV * r] * wl[r];
-#pragma unroll
-    for (int j = 0; j < NFE; ++j) {
-      const int la = loc[jj + NPV * j], lb = loc[ii + NPV * j];
-      if (la >= 0 && lb >= 0)
-        for (int g = 0; g < NGF; ++g)
-          v += sff[la + NPF * lb + NPF * NPF * g] * sFHu[j * NGF + g + NFG * (c + NCU * k)];
-    }
-    BD[idx] = v;
   }
   // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327)
   double* F = A.F + e * (int64_t)NU * NH;
@@ -443,10 +428,11 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
     rest /= NCU;
     const int k = rest % NCU;
     const int bj = rest / NCU, j = bj / NPF, b = bj - j * NPF;
-    const int la = loc[va + NPV * j];
+    const int la = sLoc[va + NPV * j];
     double v = 0.0;
     if (la >= 0)
-      for (int g = 0; g < NGF; ++g) v += sff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
+#pragma unroll
+      for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
     F[idx] = v;
   }
   // ---- 3e. GK (ncu,ndf,npv,nc) (faceint.m:342-358)
@@ -458,10 +444,11 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
     rest /= NDF;
     const int va = rest % NPV, k = rest / NPV;
     const int j = bj / NPF, b = bj - j * NPF;
-    const int la = loc[va + NPV * j];
+    const int la = sLoc[va + NPV * j];
     double v = 0.0;
     if (la >= 0)
-      for (int g = 0; g < NGF; ++g) v += sff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
+#pragma unroll
+      for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
     GK[idx] = v;
   }
   // ---- 3f. H (ncu,ndf,ncu,ndf), block diagonal per face (faceint.m:347-359)
@@ -475,7 +462,8 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
     const int j = aj / NPF, a = aj - j * NPF, j2 = bj / NPF, b = bj - j2 * NPF;
     double v = 0.0;
     if (j == j2)
-      for (int g = 0; g < NGF; ++g) v += sff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
+#pragma unroll
+      for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
     H[idx] = v;
   }
   if constexpr (UQ) {
@@ -501,10 +489,14 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
       int rest = idx / NPV;
       const int bj = rest % NDF, i = rest / NDF;
       const int j = bj / NPF, b = bj - j * NPF;
-      const int la = loc[va + NPV * j];
+      const int la = sLoc[va + NPV * j];
       double v = 0.0;
       if (la >= 0)
-        for (int g = 0; g < NGF; ++g) v += sff[la + NPF * b + NPF * NPF * g] * sNJ[j * NGF + g + NFG * i];
+#pragma unroll
+        for (int g = 0; g < NGF; ++g) {
+          const int t = j * NGF + g;
+          v += sSff[la + NPF * b + NPF * NPF * g] * (sNl[t + NFG * i] * sFJ[t]);
+        }
       sE[idx] = v;
       A.E[e * NPV * NDF * ND + idx] = v;
     }
