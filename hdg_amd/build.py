@@ -57,8 +57,9 @@ def build(changed_only=False, jobs=None):
         cmd = [HIPCC, *FLAGS, f"-DHDGX_PHYS={ph}", f"-DHDGX_ND={nd}", f"-DHDGX_P={p}", "-c",
                os.path.join(CSRC, "hdgx_inst.hip"), "-o", obj]
         jobs_list.append((obj, cmd))
-    api = os.path.join(BUILD, "hdgx_api.o")
-    jobs_list.append((api, [HIPCC, *FLAGS, "-c", os.path.join(CSRC, "hdgx_api.hip"), "-o", api]))
+    for name in ("hdgx_api", "trace"):
+        obj = os.path.join(BUILD, f"{name}.o")
+        jobs_list.append((obj, [HIPCC, *FLAGS, "-c", os.path.join(CSRC, f"{name}.hip"), "-o", obj]))
     todo = [(o, c) for o, c in jobs_list if not changed_only or _stale(o)]
     n = jobs or min(len(todo) or 1, max(1, (os.cpu_count() or 2)), 16)
     with cf.ThreadPoolExecutor(n) as ex:

@@ -10,40 +10,15 @@
 #include <vector>
 
 #include "../../include/hdgx.h"
+#include "hdgx_ctx.h"
 #include "hdgx_kernels.h"
 
 #include "hdgx_launch.h"
 
 using namespace hdgx;
-
-struct MasterBuf {
-  void* dev = nullptr;
-  DevMaster tab{};
-  int nd = 0, porder = 0, npv = 0, npf = 0, nfe = 0, ngv = 0, ngf = 0;
-};
-
-struct hdgx_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  int prec = HDGX_PREC_FP64;
-  std::string err;
-  std::map<uint64_t, MasterBuf> masters;
-  void* ws = nullptr;
-  size_t ws_bytes = 0;
-};
+using hdgx_detail::fail;
 
 namespace {
-
-int fail(hdgx_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-#define HIP_TRY(ctx, call)                                                       \
-  do {                                                                           \
-    hipError_t _e = (call);                                                      \
-    if (_e != hipSuccess) return fail(ctx, HDGX_ERR_DEVICE, std::string(#call) + ": " + hipGetErrorString(_e)); \
-  } while (0)
 
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
   const unsigned char* c = static_cast<const unsigned char*>(p);
@@ -216,6 +191,8 @@ int hdgx_destroy(hdgx_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   for (auto& kv : ctx->masters) (void)hipFree(kv.second.dev);
   if (ctx->ws) (void)hipFree(ctx->ws);
+  if (ctx->tws) (void)hipFree(ctx->tws);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
   return HDGX_OK;
 }

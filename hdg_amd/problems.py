@@ -24,6 +24,7 @@ class Problem:
     uh: np.ndarray       # (ncu, npf*nfe, ne) gathered
     sh: np.ndarray       # (npv, nc, ne)
     name: str = ""
+    UH: np.ndarray = None  # (ncu, nsiz) global trace (uh is UH gathered per element)
 
     @property
     def ne(self):
@@ -115,7 +116,7 @@ def _finish(master, app, mesh, udg, name, tdep_sh=None):
     uh = gather_uh(uhg, mesh.elcon)
     sh = np.zeros_like(udg) if tdep_sh is None else tdep_sh
     return Problem(master=master, app=app, mesh=mesh, udg=np.asfortranarray(udg), uh=uh,
-                   sh=np.asfortranarray(sh), name=name)
+                   sh=np.asfortranarray(sh), name=name, UH=uhg)
 
 
 def euler2d_square(n, porder, seed=0, minf=0.5, jitter_amp=0.1, alpha=0.0):

@@ -47,12 +47,23 @@ def lib():
         L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
+        L.hdgx_trace_assemble.argtypes = [vp] * 7
+        L.hdgx_trace_matvec.argtypes = [vp] * 5
+        L.hdgx_bilu0_factor.argtypes = [vp] * 5
+        L.hdgx_bilu0_apply.argtypes = [vp] * 5
+        L.hdgx_gmres.argtypes = [vp] * 6 + [i32, ctypes.c_double, i32, vp, vp, vp]
+        L.hdgx_trace_gather.argtypes = [vp, i32, i32, i32, i64, vp, vp, vp]
+        L.hdgx_local_update.argtypes = [vp, i32, i32, i32, i32, i32, i64, vp, vp, vp, vp, vp]
+        L.hdgx_newton_update.argtypes = [vp, i32, i32, i32, i64, i64, ctypes.c_double, vp, vp, vp, vp, vp, vp]
+        L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
         _lib = L
     return _lib
 
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
-           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq"]
+           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq",
+           "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
+           "hdgx_trace_gather", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
 
 
 def _torch():
@@ -91,6 +102,12 @@ class Dev:
         n = int(np.prod(shape)) if len(shape) else 1
         td = torch.float64 if dtype == "float64" else torch.int32
         return Dev(torch.empty(max(n, 1), dtype=td, device=device), shape)
+
+    @staticmethod
+    def zeros(shape, dtype="float64", device="cuda"):
+        d = Dev.empty(shape, dtype, device)
+        d.t.zero_()
+        return d
 
 
 def _dev(a, dtype="float64"):

@@ -131,6 +131,89 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
                   const double* Ru, const double* Rh, double* dudg, double* dudg_duh,
                   double* ae, double* fe, int32_t* status);
 
+/* ------------------------------------------------------------------------
+ * Trace system and Newton update (kernel/hdg_solve.m:105-176 with
+ * app.denseblock = 1).  The global trace unknown UH is (ncu, npf, nf): face i
+ * owns trace nodes i*npf .. i*npf+npf-1 (elcon numbering).  The system is kept
+ * as face-block rows: row i holds the (ncf x ncf) blocks, ncf = ncu*npf, of the
+ * faces that share an element with face i.  Slot 0 is face i itself, slots
+ * 1..nbf-1 are the other faces of its first element, then of its second
+ * element, in ascending local-face order (mesh/mkmesh/mkf2f.m:1-60),
+ * nbf = 2*nfe - 1.
+ * ------------------------------------------------------------------------ */
+typedef struct hdgx_trace {
+  int32_t ncu, npf, nfe;
+  int64_t ne, nf;
+  const int32_t* fconn;      /* (4, nf) device: e1, j1, e2, j2 per face (0-based; e2 = -1 on the boundary) */
+  const int32_t* f2f;        /* (nbf, nf) device: face of each block slot, -1 where none */
+  const int32_t* elcon;      /* (npf*nfe, ne) device: element trace node -> global trace node */
+  /* multicolour ordering of the face graph, used by the block ILU(0) */
+  int32_t ncolors;
+  const int32_t* color_ptr;  /* (ncolors+1) HOST: faces of colour c are color_rows[color_ptr[c] .. color_ptr[c+1]) */
+  const int32_t* color_rows; /* (nf) device */
+  const int32_t* face_rank;  /* (nf) device: position of each face in the elimination order */
+} hdgx_trace;
+
+/* Face-block system from the element matrices.  Replaces
+ *   [K,F] = mkDenseBlockSystem(AE, FE, mesh.f, mesh.t2f, mesh.elcon)
+ * (kernel/mkDenseBlockSystem.m:13-95, called by kernel/hdg_assemble.m:78-80):
+ * each block row is scaled by the inverse of its diagonal block.
+ *   AE (ncu,ndf,ncu,ndf,ne), FE (ncu,ndf,ne) ->
+ *   K (ncf, ncf, nbf-1, nf) (= the reference's (ncf, ncf*(nbf-1), nf)), F (ncf, nf),
+ *   status (nf) int32: HDGX_ST_SINGULAR_D where the diagonal block is singular. */
+int hdgx_trace_assemble(hdgx_ctx* ctx, const hdgx_trace* T, const double* AE, const double* FE,
+                        double* K, double* F, int32_t* status);
+
+/* y = x + sum_k K_k x(f2f(k)) -- the operator of kernel/gmres.m:38-42 and its
+ * per-face block GEMV kernel/arraymul.c:53-70.  x, y (ncf, nf). */
+int hdgx_trace_matvec(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double* x,
+                      double* y);
+
+/* Block ILU(0) of the face-block system (identity diagonal) in the
+ * multicolour elimination order of T.  Replaces kernel/computeBILU0.m:1-45.
+ *   LU (ncf, ncf, nbf, nf): slot 0 the inverse of U_ii, slot k the L_ik
+ *   (earlier face) or U_ik (later face) block.  status (nf): singular U_ii. */
+int hdgx_bilu0_factor(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, double* LU,
+                      int32_t* status);
+
+/* y = (LU)^-1 x (kernel/applyBILU0.m:1-55).  x, y (ncf, nf), may not alias. */
+int hdgx_bilu0_apply(hdgx_ctx* ctx, const hdgx_trace* T, const double* LU, const double* x,
+                     double* y);
+
+/* GMRES(restart) on the face-block system, FP64, classical Gram-Schmidt
+ * (kernel/gmres.m:1-111); with LU != NULL left-preconditioned by the block
+ * ILU(0) (kernel/gmres_bilu0.m:1-103).  x (ncf, nf) holds the initial guess
+ * and receives the solution.  Synchronous: returns when converged.
+ * iters / relres / flag (0 converged, 1 maxit) may be NULL. */
+int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double* LU,
+               const double* F, double* x, int restart, double tol, int maxit, int* iters,
+               double* relres, int* flag);
+
+/* UHE(:, :, e) = UH(:, elcon(:, e)): the element-gathered trace of
+ * kernel/hdg_assemble.m:34 and kernel/hdg_solve.m:120.
+ *   UH (ncu, nsiz), UHE (ncu, npf*nfe, ne). */
+int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, const int32_t* elcon,
+                      const double* UH, double* UHE);
+
+/* Local update DUDGT = DUDG + DUDG_DUH * DUH(:, elcon) (kernel/hdg_solve.m:119-121,
+ * util/mapContractK.m:47-99).  DUDG (npv,nco,ne), DUDG_DUH (npv,nco,ncu,ndf,ne),
+ * DUH (ncu, nsiz), DUDGT (npv,nco,ne). */
+int hdgx_local_update(hdgx_ctx* ctx, int npv, int nco, int ncu, int npf, int nfe, int64_t ne,
+                      const int32_t* elcon, const double* DUDG, const double* DUDG_DUH,
+                      const double* DUH, double* DUDGT);
+
+/* Damped Newton step (kernel/hdg_solve.m:145-146):
+ *   UDG(:, 1:nco, :) = UDG0(:, 1:nco, :) + alpha*DUDGT,  UH = UH0 + alpha*DUH.
+ * UDG, UDG0 (npv, nc, ne); DUDGT (npv, nco, ne); UH, UH0, DUH (nuh).  UDG may
+ * alias UDG0 and UH may alias UH0. */
+int hdgx_newton_update(hdgx_ctx* ctx, int npv, int nc, int nco, int64_t ne, int64_t nuh,
+                       double alpha, const double* UDG0, const double* DUDGT, const double* UH0,
+                       const double* DUH, double* UDG, double* UH);
+
+/* Euclidean norm of a device vector, synchronous (norm(F(:)) of
+ * kernel/hdg_solve.m:105,154). */
+int hdgx_norm2(hdgx_ctx* ctx, int64_t n, const double* x, double* result);
+
 #ifdef __cplusplus
 }
 #endif
