@@ -111,6 +111,49 @@ def cpu_baseline(P, nthreads):
     return out, P.ne / t_all, ns / t_one, kind_build, sum(runs) + sum(r1)
 
 
+def serving_leg(nreq, seed):
+    """Poisson stream of BDF1 Navier-Stokes Newton steps through the
+    continuous-batching loop: p50/p99 latency and throughput."""
+    from hdg_amd.serve import PayloadFactory, Server, make_workload
+    specs = make_workload(nreq, seed=seed, rate=50.0)
+    fac = PayloadFactory()
+    t0 = time.perf_counter()
+    for sp in specs:                       # request messages, prepared before the clock starts
+        fac.mesh(sp.n, sp.p, sp.physics)
+    log(f"[bench] serving: {nreq} requests, meshes ready in {time.perf_counter() - t0:.1f}s")
+    srv = Server(workers=4, slots=64)
+    stats, _ = srv.run(specs, factory=fac)
+    log(f"[bench] serving: p50 {stats['p50_ms']:.1f} ms, p99 {stats['p99_ms']:.1f} ms, "
+        f"{stats['requests_per_s']:.1f} req/s, {stats['elements_per_s']:.3g} el/s")
+    stats["workload"] = (f"{nreq} Poisson(50/s) BDF1 NS Newton steps, 2n^2 triangles with ne log-uniform "
+                         f"500..50000 (16 mesh sizes), p uniform 1..4, M uniform 0.2..0.7, GMRES(20)+BILU0 tol 1e-6")
+    return stats
+
+
+def tets_leg(torch, n, seed, steps):
+    """3D Euler p=2 on 6 n^3 jittered tetrahedra (u-only, FP64): elemental
+    throughput with the outputs resident in HBM."""
+    from hdg_amd.problems import euler3d_cube
+    from hdg_amd.runtime import Context, Dev
+    t0 = time.perf_counter()
+    P = euler3d_cube(n, 2, seed=seed)
+    log(f"[bench] tets: {P.ne} elements built in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+           Dev.from_numpy(P.uh), None)
+    npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.ndf, P.ne
+    out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+           Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+    ms, wall = time_steps(torch, ctx.stream, lambda: ctx.elemental(P.master, P.app, *dev, out=out), steps, 1)
+    bad = int(np.count_nonzero(out[4].numpy()))
+    res = {"workload": f"euler3d_hdg_elemental_p2_6x{n}^3", "elements": ne, "ms_per_step": ms,
+           "value": ne / (ms * 1e-3), "unit": "elements/s", "status_nonzero": bad}
+    log(f"[bench] tets: {ms:.2f} ms/step, {ne / ms * 1e3:.4g} elements/s")
+    del out, dev
+    torch.cuda.empty_cache()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -118,6 +161,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128, help="cells per side (2*n*n triangles)")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--serve-requests", type=int, default=96,
+                    help="requests of the Poisson serving workload (0 skips it)")
+    ap.add_argument("--tets", type=int, default=40, help="cube cells per side of the 3D Euler line (0 skips it)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the elemental step needs no communication and every box here has "
@@ -207,6 +253,12 @@ def main():
                          "single_core_value": cpu_rate1, "single_core_sample": f"first {min(2048, ne)} elements"},
         "speedup_vs_cpu": value / cpu_rate,
     }
+    del out, dev
+    torch.cuda.empty_cache()
+    if a.serve_requests > 0:
+        line["serving"] = serving_leg(a.serve_requests, a.seed)
+    if a.tets > 0:
+        line["tets"] = tets_leg(torch, a.tets, a.seed, min(a.steps, 5))
     print(json.dumps(line), flush=True)
 
 

@@ -192,6 +192,7 @@ int hdgx_destroy(hdgx_ctx* ctx) {
   for (auto& kv : ctx->masters) (void)hipFree(kv.second.dev);
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->tws) (void)hipFree(ctx->tws);
+  if (ctx->segdev) (void)hipFree(ctx->segdev);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
   return HDGX_OK;
@@ -327,6 +328,89 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
       int r = launch_quad<Cc>(ctx, *mb, ap, qa);
       if (r) return r;
       return launch_schur<Cc>(ctx, sa, mb->tab.perm);
+    });
+    if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+    if (rc) return rc;
+  }
+  return HDGX_OK;
+}
+
+int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, const hdgx_segment* segs) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (nseg < 0 || (nseg > 0 && !segs) || !master) return fail(ctx, HDGX_ERR_ARG, "batch: arguments");
+  if (nseg == 0) return HDGX_OK;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  const hdgx_app* a0 = segs[0].app;
+  if (!a0) return fail(ctx, HDGX_ERR_ARG, "batch: null app");
+  std::vector<ElemSeg> tab(nseg);
+  std::vector<DevApp> apps(nseg);
+  int64_t total = 0;
+  for (int s = 0; s < nseg; ++s) {
+    const hdgx_segment& g = segs[s];
+    if (!g.app) return fail(ctx, HDGX_ERR_ARG, "batch: null app");
+    const hdgx_app* a = g.app;
+    if (a->physics != a0->physics || a->nd != a0->nd || a->ncu != a0->ncu || a->nc != a0->nc ||
+        a->localsolve != a0->localsolve)
+      return fail(ctx, HDGX_ERR_ARG, "batch: segments must share physics, nd, ncu, nc and localsolve");
+    if ((rc = make_app(ctx, master, a, &apps[s]))) return rc;
+    if (g.ne < 0) return fail(ctx, HDGX_ERR_ARG, "batch: ne < 0");
+    if (g.ne > 0 && (!g.dgnodes || !g.bf || !g.udg || !g.uh || !g.ae || !g.fe || !g.dudg || !g.dudg_duh))
+      return fail(ctx, HDGX_ERR_ARG, "batch: null array");
+    if (g.ne > 0 && a->tdep && !g.sh) return fail(ctx, HDGX_ERR_ARG, "batch: tdep needs sh");
+    tab[s] = ElemSeg{total, g.ne, g.dgnodes, g.bf, g.udg, g.uh, g.sh, g.ae, g.fe, g.dudg, g.dudg_duh, g.status, s, 0};
+    total += g.ne;
+  }
+  if (total == 0) return HDGX_OK;
+  const int nd = master->nd, npv = master->npv, ncu = a0->ncu, nc = a0->nc;
+  const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
+  const bool uq = a0->localsolve != 0;
+  const int p = porder_of(nd, npv);
+  const size_t tbytes = sizeof(ElemSeg) * nseg + sizeof(DevApp) * nseg;
+  if (ctx->segdev_bytes < tbytes) {
+    if (ctx->segdev) HIP_TRY(ctx, hipFree(ctx->segdev));
+    ctx->segdev = nullptr;
+    ctx->segdev_bytes = 0;
+    HIP_TRY(ctx, hipMalloc(&ctx->segdev, tbytes));
+    ctx->segdev_bytes = tbytes;
+  }
+  ElemSeg* dsegs = static_cast<ElemSeg*>(ctx->segdev);
+  DevApp* dapps = reinterpret_cast<DevApp*>(dsegs + nseg);
+  HIP_TRY(ctx, hipMemcpyAsync(dsegs, tab.data(), sizeof(ElemSeg) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  HIP_TRY(ctx, hipMemcpyAsync(dapps, apps.data(), sizeof(DevApp) * nseg, hipMemcpyHostToDevice, ctx->stream));
+  const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
+                     (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
+  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)((size_t)6 << 30) / (int64_t)(per * 8)));
+  if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
+  for (int64_t g0 = 0; g0 < total; g0 += chunk) {
+    const int64_t n = std::min(chunk, total - g0);
+    double* w = static_cast<double*>(ctx->ws);
+    SegTable t{};
+    t.segs = dsegs;
+    t.apps = dapps;
+    t.nseg = nseg;
+    t.g0 = g0;
+    t.total = n;
+    t.BD = w; w += (size_t)nu * ncol * n;
+    t.F = w; w += (size_t)nu * nh * n;
+    t.GK = w; w += (size_t)nh * ncol * n;
+    t.H = w; w += (size_t)nh * nh * n;
+    t.Ru = w; w += (size_t)nu * n;
+    t.Rh = w; w += (size_t)nh * n;
+    if (uq) {
+      t.M = w; w += (size_t)npv * npv * n;
+      t.C = w; w += (size_t)npv * npv * nd * n;
+      t.E = w; w += (size_t)npv * ndf * nd * n;
+      t.Rq = w; w += (size_t)npv * ncu * nd * n;
+    }
+    rc = with_cfg(a0->physics, nd, p, [&](auto cfg) {
+      using Cc = decltype(cfg);
+      if (mb->npv != Cc::NPV || mb->npf != Cc::NPF || mb->ngv != Cc::NGV || mb->ngf != Cc::NGF)
+        return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled configuration");
+      HIP_TRY(ctx, seg_launch<Cc>(ctx->stream, t, mb->tab, mb->tab.perm));
+      return (int)HDGX_OK;
     });
     if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
     if (rc) return rc;

@@ -28,6 +28,8 @@ struct hdgx_ctx {
   void* tws = nullptr;           // trace-solve workspace (Krylov basis, reductions)
   size_t tws_bytes = 0;
   double* pinned = nullptr;      // small host staging for reductions
+  void* segdev = nullptr;        // segment and physics tables of batched launches
+  size_t segdev_bytes = 0;
 };
 
 namespace hdgx_detail {

@@ -6,6 +6,7 @@
 #include "quad_kernel.hip"
 #include "schur_kernel.hip"
 #include "schur_mfma.hip"
+#include "seg_kernels.hip"
 
 namespace hdgx {
 
@@ -35,8 +36,19 @@ hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) 
   return hipGetLastError();
 }
 
+template <class C>
+hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, const int32_t* perm) {
+  if (t.total <= 0) return hipSuccess;
+  hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QUAD_THREADS), 0, s, t, m);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm);
+  return hipGetLastError();
+}
+
 using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
 template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
 template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*);
+template hipError_t seg_launch<ThisCfg>(hipStream_t, const SegTable&, const DevMaster&, const int32_t*);
 
 }  // namespace hdgx

@@ -72,4 +72,30 @@ struct SchurArgs {
   int32_t prec;                    // 0 fp64, 1 fp32 + fp64 refinement
 };
 
+// One request's element range inside a batched launch: its arrays and the
+// index of its physics parameters in the launch's DevApp table.
+struct ElemSeg {
+  int64_t e0, ne;                  // first element in the batch numbering, count
+  const double* dgnodes;
+  const int32_t* bf;
+  const double* udg;
+  const double* uh;
+  const double* sh;
+  double *ae, *fe, *dudg, *dudg_duh;
+  int32_t* status;
+  int32_t app;
+  int32_t pad;
+};
+
+// Segment table of one launch over batch elements g0 .. g0+total-1; the
+// element matrices between the two kernels live in the workspace arrays,
+// indexed by g - g0.
+struct SegTable {
+  const ElemSeg* segs;
+  const DevApp* apps;
+  int32_t nseg;
+  int64_t g0, total;
+  double *BD, *F, *GK, *H, *Ru, *Rh, *M, *C, *E, *Rq;
+};
+
 }  // namespace hdgx

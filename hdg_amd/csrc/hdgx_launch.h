@@ -19,11 +19,15 @@ template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, cons
 // prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 + FP64 refinement; 2: FP64 row-per-lane LU.
 // perm (device, may be null) marks GK as face sparse.
 template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm);
+// batched launches over a segment table (FP64 MFMA elimination)
+template <class C> hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, const int32_t* perm);
 
 #define HDGX_DECLARE(PH, ND, P)                                                                          \
   extern template hipError_t quad_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
                                                          const DevApp&);                                  \
-  extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&, const int32_t*);
+  extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&, const int32_t*); \
+  extern template hipError_t seg_launch<Cfg<PH, ND, P>>(hipStream_t, const SegTable&, const DevMaster&,  \
+                                                        const int32_t*);
 HDGX_FOR_EACH_CFG(HDGX_DECLARE)
 #undef HDGX_DECLARE
 

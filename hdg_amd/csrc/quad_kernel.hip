@@ -118,8 +118,9 @@ HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const doub
   }
 }
 
+// Element e of the batch A (a workgroup of QUAD_THREADS threads).
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
   constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
@@ -127,7 +128,6 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
   constexpr bool UQ = C::UQ;
 
   const int tid = threadIdx.x;
-  const int64_t e = blockIdx.x;
   if (e >= A.ne) return;
   const int ncd = ap.ncd;
   const bool tdep = ap.tdep != 0;
@@ -514,6 +514,11 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaste
       A.Rq[e * NPV * NCU * ND + idx] = v + w - z;
     }
   }
+}
+
+template <class C>
+__global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+  quad_body<C>(A, T, ap, blockIdx.x);
 }
 
 }  // namespace hdgx

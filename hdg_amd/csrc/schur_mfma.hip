@@ -78,11 +78,9 @@ HDGX_DEV constexpr bool bmie_needed(int q0, int s) {
   return false;
 }
 
+// Element e of the batch A (one 64-lane wavefront).
 template <class C>
-// two wavefronts per SIMD where the system fits: the compiler then keeps the
-// accumulator tiles in VGPRs within 256 registers (15% faster on NS p=3 than
-// one wavefront per SIMD with AGPR accumulators)
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e) {
   using S = MfmaShape<C>;
   constexpr int NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, NCU = C::NCU, ND = C::ND;
   constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NC = C::NC;
@@ -94,7 +92,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
 
   const int lane = threadIdx.x;
   const int g = lane >> 4, c = lane & 15;
-  const int64_t e = blockIdx.x;
   if (e >= A.ne) return;
 
   // strides padded so that lanes of a wavefront hit distinct LDS banks
@@ -481,6 +478,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
   }
   st = wave_or(st);
   if (lane == 0 && A.status) A.status[e] = st;
+}
+
+// two wavefronts per SIMD where the system fits: the compiler then keeps the
+// accumulator tiles in VGPRs within 256 registers (15% faster on NS p=3 than
+// one wavefront per SIMD with AGPR accumulators)
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+  schur_mfma_body<C>(A, perm, blockIdx.x);
 }
 
 }  // namespace hdgx

@@ -1,0 +1,103 @@
+// Batched launches over the element ranges of many requests: one workgroup
+// per batch element finds its request by binary search in the segment table
+// and runs the same per-element bodies as the single-batch kernels, with its
+// own physics parameters.
+#include <hip/hip_runtime.h>
+
+#include "hdgx_device.h"
+#include "hdgx_kernels.h"
+
+namespace hdgx {
+
+// wavefront-uniform loads of the tables through the scalar cache
+template <class T>
+HDGX_DEV T uload_obj(const T* p) {
+  static_assert(sizeof(T) % 8 == 0, "tables are loaded in 8-byte words");
+  T r;
+  const __attribute__((address_space(4))) uint64_t* q = (const __attribute__((address_space(4))) uint64_t*)p;
+  uint64_t* d = reinterpret_cast<uint64_t*>(&r);
+#pragma unroll
+  for (unsigned i = 0; i < sizeof(T) / 8; ++i) d[i] = q[i];
+  return r;
+}
+
+HDGX_DEV int find_seg(const SegTable& S, int64_t g) {
+  int lo = 0, hi = S.nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    const int64_t e0 = *((const __attribute__((address_space(4))) int64_t*)&S.segs[mid].e0);
+    if (e0 <= g) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <class C>
+__global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevMaster T) {
+  constexpr int NPV = C::NPV, NFE = C::NFE, NC = C::NC, NU = C::NU, NH = C::NH, NCOL = C::NCOL;
+  constexpr int ND = C::ND, NDF = C::NDF, NCU = C::NCU;
+  const int64_t w = blockIdx.x;
+  if (w >= S.total) return;
+  const int64_t g = S.g0 + w;
+  const ElemSeg sg = uload_obj(S.segs + find_seg(S, g));
+  const DevApp& ap = S.apps[sg.app];
+  const int64_t e = g - sg.e0;
+  QuadArgs A;
+  A.ne = 1;
+  A.dgnodes = sg.dgnodes + e * NPV * ap.ncd;
+  A.bf = sg.bf + e * NFE;
+  A.udg = sg.udg + e * NPV * NC;
+  A.uh = sg.uh + e * NH;
+  A.sh = sg.sh ? sg.sh + e * NPV * NC : nullptr;
+  A.BD = S.BD + w * NU * NCOL;
+  A.F = S.F + w * NU * NH;
+  A.GK = S.GK + w * NH * NCOL;
+  A.H = S.H + w * NH * NH;
+  A.Ru = S.Ru + w * NU;
+  A.Rh = S.Rh + w * NH;
+  if constexpr (C::UQ) {
+    A.M = S.M + w * NPV * NPV;
+    A.C = S.C + w * NPV * NPV * ND;
+    A.E = S.E + w * NPV * NDF * ND;
+    A.Rq = S.Rq + w * NPV * NCU * ND;
+  } else {
+    A.M = A.C = A.E = A.Rq = nullptr;
+  }
+  A.status = nullptr;
+  quad_body<C>(A, T, ap, 0);
+}
+
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm) {
+  constexpr int NPV = C::NPV, NU = C::NU, NH = C::NH, NCOL = C::NCOL, ND = C::ND, NDF = C::NDF, NCU = C::NCU;
+  const int64_t w = blockIdx.x;
+  if (w >= S.total) return;
+  const int64_t g = S.g0 + w;
+  const ElemSeg sg = uload_obj(S.segs + find_seg(S, g));
+  const int64_t e = g - sg.e0;
+  SchurArgs A;
+  A.ne = 1;
+  A.BD = S.BD + w * NU * NCOL;
+  A.F = S.F + w * NU * NH;
+  A.GK = S.GK + w * NH * NCOL;
+  A.H = S.H + w * NH * NH;
+  A.Ru = S.Ru + w * NU;
+  A.Rh = S.Rh + w * NH;
+  if constexpr (C::UQ) {
+    A.M = S.M + w * NPV * NPV;
+    A.C = S.C + w * NPV * NPV * ND;
+    A.E = S.E + w * NPV * NDF * ND;
+    A.Rq = S.Rq + w * NPV * NCU * ND;
+  } else {
+    A.M = A.C = A.E = A.Rq = nullptr;
+  }
+  A.dudg = sg.dudg + e * NCOL;
+  A.dudg_duh = sg.dudg_duh + e * NCOL * NH;
+  A.ae = sg.ae + e * NH * NH;
+  A.fe = sg.fe + e * NH;
+  A.status = sg.status ? sg.status + e : nullptr;
+  A.prec = 0;
+  schur_mfma_body<C>(A, perm, 0);
+}
+
+}  // namespace hdgx

@@ -151,3 +151,20 @@ def euler3d_cube(n, porder=2, seed=0, minf=0.4, jitter_amp=0.1, alpha=0.0):
     mesh = mkmesh_cube(n, m, jitter_amp, seed)
     udg = euler_state(mesh.dgnodes, 3, 1.4, minf, seed + 1, bump_p=False, alpha=alpha)
     return _finish(m, app, mesh, udg, f"euler3d_p{porder}_n{n}")
+
+
+def with_bdf1(P, dt, seed=0, amp=0.01):
+    """Turns P into one BDF1 step (kernel/hdg_solve_bdf.m:29-43, torder 1):
+    tdep = 1, fc_u = 1/dt, SH = u_n/dt with the previous state u_n a seeded
+    perturbation of P.udg (q components of SH zero)."""
+    rng = _rng(seed)
+    ncu = P.app.ncu
+    un = P.udg[:, :ncu, :] * (1.0 + amp * rng.standard_normal(P.udg[:, :ncu, :].shape))
+    sh = np.zeros_like(P.udg)
+    sh[:, :ncu, :] = un / dt
+    P.app.tdep = 1
+    P.app.fc_u = 1.0 / dt
+    P.sh = np.asfortranarray(sh)
+    P.name += "_bdf1"
+    return P
+

@@ -47,6 +47,7 @@ def lib():
         L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
+        L.hdgx_elemental_batch.argtypes = [vp, vp, i32, vp]
         L.hdgx_trace_assemble.argtypes = [vp] * 7
         L.hdgx_trace_matvec.argtypes = [vp] * 5
         L.hdgx_bilu0_factor.argtypes = [vp] * 5
@@ -61,7 +62,7 @@ def lib():
 
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
-           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq",
+           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq", "hdgx_elemental_batch",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
 
@@ -120,6 +121,11 @@ def _dev(a, dtype="float64"):
 
 def _p(a):
     return None if a is None else a.ptr
+
+
+class CSegment(ctypes.Structure):
+    _fields_ = [("ne", ctypes.c_int64)] + [(n, ctypes.c_void_p) for n in (
+        "dgnodes", "bf", "udg", "uh", "sh", "ae", "fe", "dudg", "dudg_duh", "status", "app")]
 
 
 class Context:
@@ -184,6 +190,23 @@ class Context:
         self._check(lib().hdgx_elemental(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
                                          h.ptr, _p(s), ae.ptr, fe.ptr, dudg.ptr, dudg_duh.ptr, st.ptr))
         return ae, fe, dudg, dudg_duh, st
+
+    def elemental_batch(self, master, segments):
+        """One batched elemental launch over many requests' element ranges.
+        segments: list of (app, dgnodes, bf, udg, uh, sh, (ae, fe, dudg, dudg_duh, status)) with
+        Dev arrays; every request keeps its own physics parameters."""
+        mc = self.master_c(master)
+        keep = []
+        arr = (CSegment * max(1, len(segments)))()
+        for k, (app, dg, b, u, h, s, out) in enumerate(segments):
+            ca = app.to_c() if isinstance(app, App) else app
+            keep.append(ca)
+            ae, fe, dudg, dudg_duh, st = out
+            arr[k] = CSegment(dg.shape[2], dg.ptr.value, b.ptr.value, u.ptr.value, h.ptr.value,
+                              s.ptr.value if s is not None else None, ae.ptr.value, fe.ptr.value, dudg.ptr.value,
+                              dudg_duh.ptr.value, st.ptr.value if st is not None else None,
+                              ctypes.cast(ctypes.pointer(ca), ctypes.c_void_p).value)
+        self._check(lib().hdgx_elemental_batch(self.h, ctypes.byref(mc.c), len(segments), arr))
 
     def elemental_matrices(self, master, app, dgnodes, bf, udg, uh, sh=None):
         ca = app.to_c() if isinstance(app, App) else app

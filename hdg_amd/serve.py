@@ -1,0 +1,408 @@
+"""Device-resident serving loop for implicit HDG steps, with continuous batching.
+
+Requests are independent HDG problems (a mesh, an order p, a state) that each
+need one or more Newton iterations of an implicit (BDF1) or steady solve
+(kernel/hdg_solve.m:103-170, kernel/hdg_solve_bdf.m:26-48).  The loop keeps
+every admitted request's state in HBM and advances all of them together:
+
+  * admission: a request that has arrived is admitted as soon as its HBM
+    footprint fits the budget and a slot is free;
+  * elemental phase: the requests waiting for an elemental evaluation are
+    grouped by order p and each group runs as ONE batched launch
+    (hdgx_elemental_batch), every request with its own physics parameters;
+  * trace phase: each request's face-block system, block ILU(0), GMRES and
+    local update run on a worker stream (hdgx_trace_*, hdgx_gmres,
+    hdgx_local_update), so the trace solves of several requests and the next
+    elemental batch overlap on the device;
+  * a request whose iterations are done frees its memory; one with more
+    iterations rejoins the next elemental batch.
+
+Latency of a request = completion time - arrival time (wall clock).
+"""
+import argparse
+import json
+import os
+import queue
+import sys
+import threading
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .runtime import Context, Dev, HdgxError, _torch
+from .trace import TraceSystem
+
+PHYS_NS = "ns"
+PHYS_EULER = "euler"
+
+
+@dataclass
+class RequestSpec:
+    rid: int
+    t_arrival: float
+    n: int                 # cells per side: 2*n*n triangles
+    p: int
+    minf: float
+    seed: int
+    dt: float = 0.05
+    physics: str = PHYS_NS
+    iters: int = 1         # Newton iterations (1 = one BDF1 Newton step)
+
+    @property
+    def ne(self):
+        return 2 * self.n * self.n
+
+
+def make_workload(nreq=512, seed=2024, rate=50.0, ne_lo=500, ne_hi=50000, levels=16, physics=PHYS_NS,
+                  dt=0.05):
+    """Poisson arrivals at `rate` per second; ne log-uniform in [ne_lo, ne_hi]
+    snapped to `levels` log-spaced square meshes (2*n*n triangles), p uniform
+    in 1..4, M uniform in [0.2, 0.7], all from the seed."""
+    rng = np.random.default_rng(np.random.SeedSequence(int(seed)))
+    t = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
+    ne = np.exp(rng.uniform(np.log(ne_lo), np.log(ne_hi), size=nreq))
+    lv = np.unique(np.round(np.geomspace(np.sqrt(ne_lo / 2), np.sqrt(ne_hi / 2), levels)).astype(int))
+    n = lv[np.argmin(np.abs(np.log(lv[None, :]) - np.log(np.sqrt(ne / 2))[:, None]), axis=1)]
+    p = rng.integers(1, 5, size=nreq)
+    minf = rng.uniform(0.2, 0.7, size=nreq)
+    seeds = rng.integers(0, 2 ** 31, size=nreq)
+    return [RequestSpec(i, float(t[i]), int(n[i]), int(p[i]), float(minf[i]), int(seeds[i]), dt, physics)
+            for i in range(nreq)]
+
+
+class PayloadFactory:
+    """Host inputs of a request (the request's message): mesh and trace
+    connectivity cached per (n, p), state from the request's seed."""
+
+    def __init__(self):
+        self._mesh = {}
+        self._master = {}
+
+    def master(self, p):
+        from .master import mkmaster
+        if p not in self._master:
+            self._master[p] = mkmaster(2, p)
+        return self._master[p]
+
+    def mesh(self, n, p, physics):
+        from .mesh import mkmesh_square
+        from .trace import color_faces, face_connectivity, mkf2f
+        key = (n, p, physics)
+        if key not in self._mesh:
+            m = self.master(p)
+            mesh = mkmesh_square(n, n, m, 0.1, seed=1000 + n, ncd_extra=0)
+            f2f = mkf2f(mesh)
+            conn = dict(fconn=face_connectivity(mesh), f2f=f2f, color=color_faces(f2f))
+            self._mesh[key] = (mesh, conn)
+        return self._mesh[key]
+
+    def build(self, spec):
+        from .app import euler_app, ns_app
+        from .problems import euler_state, inituhat, ns_state
+        m = self.master(spec.p)
+        mesh, conn = self.mesh(spec.n, spec.p, spec.physics)
+        if spec.physics == PHYS_NS:
+            app = ns_app(minf=spec.minf)
+            udg = ns_state(mesh.dgnodes, 1.4, spec.minf, spec.seed)
+        else:
+            app = euler_app(2, minf=spec.minf)
+            udg = euler_state(mesh.dgnodes, 2, 1.4, spec.minf, spec.seed)
+        UH = inituhat(m, mesh.elcon, udg, app.ncu, mesh.nsiz)
+        sh = np.zeros_like(udg)
+        if spec.dt > 0:
+            # BDF1 (hdg_solve_bdf.m:29-43 with torder 1): fc_u = 1/dt, SH = u_n/dt
+            app.tdep = 1
+            app.fc_u = 1.0 / spec.dt
+            sh[:, :app.ncu, :] = udg[:, :app.ncu, :] / spec.dt
+        return dict(master=m, app=app, mesh=mesh, conn=conn, udg=np.asfortranarray(udg),
+                    UH=np.asfortranarray(UH), sh=np.asfortranarray(sh))
+
+
+def footprint_bytes(spec, master, app, nf):
+    """HBM bytes a request holds while admitted."""
+    npv, ncu, nc = master.npv, app.ncu, app.nc
+    ndf = master.npf * master.nfe
+    ne = spec.ne
+    ncf = ncu * master.npf
+    nbf = 2 * master.nfe - 1
+    el = ne * (ncu * ndf * ncu * ndf + ncu * ndf + npv * nc + npv * nc * ncu * ndf)   # AE FE DUDG DUDG_DUH
+    st = ne * (3 * npv * nc + npv * 3 + ncu * ndf) + 4 * ncf * nf                   # UDG UDG0 SH DUDGT dgnodes uhe, UH..
+    tr = nf * (ncf * ncf * (nbf - 1) + ncf * ncf * nbf + 2 * ncf)                     # K LU F
+    return 8 * (el + st + tr)
+
+
+@dataclass
+class Slot:
+    spec: RequestSpec
+    payload: dict
+    bytes: int
+    state: object = None
+    it: int = 0
+    t_admit: float = 0.0
+    t_done: float = 0.0
+    residuals: list = field(default_factory=list)
+    gmres_iters: list = field(default_factory=list)
+    views: tuple = None
+    event: object = None
+
+
+class _ReqState:
+    """Device arrays of an admitted request (the elemental outputs are views of
+    a batch launch's buffers)."""
+
+    def __init__(self, ctx, payload, stream):
+        torch = _torch()
+        m, app, mesh = payload["master"], payload["app"], payload["mesh"]
+        conn = payload["conn"]
+        with torch.cuda.stream(stream):
+            self.ts = TraceSystem(ctx, mesh, app.ncu, with_bilu0=True, conn=conn)
+            self.dgnodes = Dev.from_numpy(mesh.dgnodes)
+            self.bf = Dev.from_numpy(mesh.bf, np.int32)
+            self.UDG = Dev.from_numpy(payload["udg"])
+            self.UH = Dev.from_numpy(payload["UH"])
+            self.SH = Dev.from_numpy(payload["sh"])
+            ts = self.ts
+            self.uhe = Dev.empty((app.ncu, m.npf * m.nfe, mesh.ne))
+            self.K = Dev.empty((ts.ncf, ts.ncf, ts.nbf - 1, ts.nf))
+            self.F = Dev.empty((ts.ncf, ts.nf))
+            self.fst = Dev.empty((ts.nf,), "int32")
+            self.LU = Dev.empty((ts.ncf, ts.ncf, ts.nbf, ts.nf))
+            self.DUH = Dev.zeros((ts.ncf, ts.nf))
+            self.DUDGT = Dev.empty((m.npv, app.nc, mesh.ne))
+
+
+class Server:
+    """Continuous-batching loop on one device.
+
+    budget_bytes: HBM the admitted requests may hold together; slots: maximum
+    number of admitted requests; workers: concurrent trace solves (each with
+    its own hdgx context and stream)."""
+
+    def __init__(self, device=0, budget_bytes=None, slots=64, workers=4, restart=20, gmres_tol=1e-6,
+                 max_batch_elems=2_000_000):
+        torch = _torch()
+        self.device = device
+        if budget_bytes is None:
+            free, total = torch.cuda.mem_get_info(device)
+            budget_bytes = int(0.6 * free)
+        self.budget = budget_bytes
+        self.slots = slots
+        self.nworkers = workers
+        self.restart, self.gmres_tol = restart, gmres_tol
+        self.max_batch_elems = max_batch_elems
+        self.ctx = Context(device)
+        self.main_stream = torch.cuda.current_stream(device)
+
+    # ------------------------------------------------------------- workers
+    def _worker(self, wq, doneq):
+        torch = _torch()
+        stream = torch.cuda.Stream(self.device)
+        ctx = Context(self.device)
+        ctx.bind_stream(stream)
+        while True:
+            item = wq.get()
+            if item is None:
+                return
+            slot = item
+            try:
+                with torch.cuda.stream(stream):
+                    stream.wait_event(slot.event)
+                    self._trace_step(ctx, slot)
+                    stream.synchronize()
+            except Exception as exc:      # surfaced by the scheduler
+                doneq.put((slot, exc))
+                continue
+            doneq.put((slot, None))
+
+    def _trace_step(self, ctx, slot):
+        S = slot.state
+        ts = S.ts
+        ts.ctx = ctx
+        AE, FE, DUDG, DUDG_DUH, est = slot.views
+        ts.assemble(AE, FE, out=(S.K, S.F, S.fst))
+        slot.residuals.append(ts.norm2(S.F))
+        ts.bilu0_factor(S.K, S.LU)
+        S.DUH.t.zero_()
+        _, it, rr, fl = ts.gmres(S.K, S.F, S.DUH, LU=S.LU, restart=self.restart, tol=self.gmres_tol)
+        slot.gmres_iters.append(it)
+        m, app = slot.payload["master"], slot.payload["app"]
+        ts.local_update(m.npv, app.nc, DUDG, DUDG_DUH, S.DUH, S.DUDGT)
+        ts.newton_update(m.npv, app.nc, app.nc, 1.0, S.UDG, S.DUDGT, S.UH, S.DUH, S.UDG, S.UH)
+        slot.it += 1
+
+    # ------------------------------------------------------------ elemental
+    def _elemental_batches(self, ready):
+        """Group ready slots by (physics, p) and launch one batch per group."""
+        torch = _torch()
+        groups = {}
+        for s in ready:
+            groups.setdefault((s.spec.physics, s.spec.p), []).append(s)
+        launched = []
+        for (phys, p), slots in groups.items():
+            # split very large groups so one batch's buffers stay bounded
+            chunk, cur = [], 0
+            for s in slots:
+                if chunk and cur + s.spec.ne > self.max_batch_elems:
+                    launched.append(self._launch_group(chunk))
+                    chunk, cur = [], 0
+                chunk.append(s)
+                cur += s.spec.ne
+            if chunk:
+                launched.append(self._launch_group(chunk))
+        return launched
+
+    def _launch_group(self, slots):
+        torch = _torch()
+        m = slots[0].payload["master"]
+        app0 = slots[0].payload["app"]
+        npv, ncu, nc = m.npv, app0.ncu, app0.nc
+        ndf = m.npf * m.nfe
+        nh = ncu * ndf
+        NE = sum(s.spec.ne for s in slots)
+        AE = torch.empty(nh * nh * NE, dtype=torch.float64, device="cuda")
+        FE = torch.empty(nh * NE, dtype=torch.float64, device="cuda")
+        DU = torch.empty(npv * nc * NE, dtype=torch.float64, device="cuda")
+        DX = torch.empty(npv * nc * nh * NE, dtype=torch.float64, device="cuda")
+        ST = torch.empty(NE, dtype=torch.int32, device="cuda")
+        segs = []
+        e0 = 0
+        for s in slots:
+            S = s.state
+            ne = s.spec.ne
+            S.ts.ctx = self.ctx
+            S.ts.gather(S.UH, S.uhe)
+            v = (Dev(AE[nh * nh * e0:nh * nh * (e0 + ne)], (ncu, ndf, ncu, ndf, ne)),
+                 Dev(FE[nh * e0:nh * (e0 + ne)], (ncu, ndf, ne)),
+                 Dev(DU[npv * nc * e0:npv * nc * (e0 + ne)], (npv, nc, ne)),
+                 Dev(DX[npv * nc * nh * e0:npv * nc * nh * (e0 + ne)], (npv, nc, ncu, ndf, ne)),
+                 Dev(ST[e0:e0 + ne], (ne,)))
+            s.views = v
+            segs.append((s.payload["app"], S.dgnodes, S.bf, S.UDG, S.uhe, S.SH, v))
+            e0 += ne
+        self.ctx.elemental_batch(m, segs)
+        ev = torch.cuda.Event()
+        ev.record(self.main_stream)
+        for s in slots:
+            s.event = ev
+        return len(slots), NE
+
+    # ----------------------------------------------------------------- loop
+    def run(self, specs, factory=None, realtime=True, log=None, keep_state=False):
+        """Serve `specs` (sorted by arrival).  realtime=False admits every
+        request at once (arrival times ignored).  Returns a stats dict and the
+        finished slots (with their device state when keep_state)."""
+        torch = _torch()
+        factory = factory or PayloadFactory()
+        pending = sorted(specs, key=lambda s: s.t_arrival)
+        payloads = {s.rid: factory.build(s) for s in pending}      # request messages, prepared off the clock
+        wq, doneq = queue.Queue(), queue.Queue()
+        threads = [threading.Thread(target=self._worker, args=(wq, doneq), daemon=True) for _ in range(self.nworkers)]
+        for t in threads:
+            t.start()
+        used = 0
+        active = []
+        ready = []              # waiting for an elemental evaluation
+        finished = []
+        nbatches = 0
+        batched_elems = 0
+        t0 = time.perf_counter()
+        clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
+        inflight = 0
+        try:
+            while pending or active:
+                now = clock()
+                # admission by arrival, slots and HBM budget
+                while pending and pending[0].t_arrival <= now and len(active) < self.slots:
+                    spec = pending[0]
+                    pl = payloads[spec.rid]
+                    b = footprint_bytes(spec, pl["master"], pl["app"], pl["mesh"].nf)
+                    if used + b > self.budget and active:
+                        break
+                    pending.pop(0)
+                    slot = Slot(spec, pl, b)
+                    slot.t_admit = time.perf_counter() - t0
+                    slot.state = _ReqState(self.ctx, pl, self.main_stream)
+                    used += b
+                    active.append(slot)
+                    ready.append(slot)
+                if ready:
+                    for n, ne in self._elemental_batches(ready):
+                        nbatches += 1
+                        batched_elems += ne
+                    for s in ready:
+                        wq.put(s)
+                        inflight += 1
+                    ready = []
+                # collect finished trace steps; wait when nothing else can progress
+                timeout = None
+                if pending and len(active) < self.slots:
+                    timeout = max(0.0, pending[0].t_arrival - clock()) if realtime else 0.0
+                if inflight == 0 and timeout is None:
+                    timeout = 0.0
+                try:
+                    item = doneq.get(timeout=timeout) if (timeout is None or timeout > 0) else doneq.get_nowait()
+                except queue.Empty:
+                    continue
+                while True:
+                    slot, exc = item
+                    inflight -= 1
+                    if exc is not None:
+                        raise exc
+                    if slot.it < slot.spec.iters:
+                        ready.append(slot)
+                    else:
+                        slot.t_done = time.perf_counter() - t0
+                        slot.views = None
+                        if not keep_state:
+                            slot.state = None
+                        active.remove(slot)
+                        used -= slot.bytes
+                        finished.append(slot)
+                        if log:
+                            log(f"[serve] req {slot.spec.rid} p={slot.spec.p} ne={slot.spec.ne} done "
+                                f"{(slot.t_done - (slot.spec.t_arrival if realtime else 0)) * 1e3:.1f} ms")
+                    try:
+                        item = doneq.get_nowait()
+                    except queue.Empty:
+                        break
+        finally:
+            for _ in threads:
+                wq.put(None)
+            for t in threads:
+                t.join()
+        makespan = time.perf_counter() - t0
+        lat = np.array([(s.t_done - (s.spec.t_arrival if realtime else 0.0)) for s in finished])
+        ne_total = sum(s.spec.ne * s.spec.iters for s in finished)
+        stats = {
+            "requests": len(finished),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3) if lat.size else None,
+            "p99_ms": float(np.percentile(lat, 99) * 1e3) if lat.size else None,
+            "mean_ms": float(lat.mean() * 1e3) if lat.size else None,
+            "makespan_s": makespan,
+            "requests_per_s": len(finished) / makespan if makespan > 0 else None,
+            "elements_per_s": ne_total / makespan if makespan > 0 else None,
+            "elemental_batches": nbatches,
+            "mean_requests_per_batch": (sum(s.spec.iters for s in finished) / nbatches) if nbatches else None,
+            "gmres_iters_mean": float(np.mean([np.mean(s.gmres_iters) for s in finished])) if finished else None,
+        }
+        return stats, finished
+
+
+def main():
+    ap = argparse.ArgumentParser(description="HDG serving loop (continuous batching) on one GPU")
+    ap.add_argument("--requests", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--rate", type=float, default=50.0)
+    ap.add_argument("--workers", type=int, default=4)
+    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--physics", default=PHYS_NS, choices=[PHYS_NS, PHYS_EULER])
+    a = ap.parse_args()
+    specs = make_workload(a.requests, a.seed, a.rate, physics=a.physics)
+    srv = Server(workers=a.workers, slots=a.slots)
+    stats, _ = srv.run(specs, log=lambda m: print(m, file=sys.stderr, flush=True))
+    print(json.dumps(stats))
+
+
+if __name__ == "__main__":
+    main()

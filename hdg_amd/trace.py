@@ -98,7 +98,9 @@ class TraceSystem:
     All vectors are ``Dev`` arrays of shape (ncf, nf) (ncf = ncu*npf): the
     layout of UH (ncu, npf*nf)."""
 
-    def __init__(self, ctx, mesh, ncu, with_bilu0=True):
+    def __init__(self, ctx, mesh, ncu, with_bilu0=True, conn=None):
+        """conn: optional host connectivity already built for this mesh
+        (dict with fconn, f2f and color = color_faces(f2f))."""
         self.ctx = ctx
         self.mesh = mesh
         npf = mesh.elcon.shape[0] // mesh.t2f.shape[1]
@@ -106,16 +108,20 @@ class TraceSystem:
         self.ne, self.nf = mesh.t.shape[0], mesh.f.shape[0]
         self.ncf = ncu * npf
         self.nbf = 2 * self.nfe - 1
-        self.fconn = Dev.from_numpy(face_connectivity(mesh), np.int32)
-        self.f2f_host = mkf2f(mesh)
+        conn = conn or {}
+        self.fconn = Dev.from_numpy(conn["fconn"] if "fconn" in conn else face_connectivity(mesh), np.int32)
+        self.f2f_host = conn["f2f"] if "f2f" in conn else mkf2f(mesh)
         self.f2f = Dev.from_numpy(self.f2f_host, np.int32)
         self.elcon = Dev.from_numpy(mesh.elcon, np.int32)
         c = CTrace(ncu, npf, self.nfe, self.ne, self.nf, self.fconn.ptr.value, self.f2f.ptr.value,
                    self.elcon.ptr.value, 0, None, None, None)
         if with_bilu0:
-            ptr, rows, rank, color = color_faces(self.f2f_host)
-            if not check_coloring(self.f2f_host, color):
-                raise HdgxError("face colouring is not proper")
+            if "color" in conn:
+                ptr, rows, rank, color = conn["color"]
+            else:
+                ptr, rows, rank, color = color_faces(self.f2f_host)
+                if not check_coloring(self.f2f_host, color):
+                    raise HdgxError("face colouring is not proper")
             self.ncolors = len(ptr) - 1
             self._color_ptr = (ctypes.c_int32 * len(ptr))(*ptr.tolist())
             self.color_rows = Dev.from_numpy(rows, np.int32)

@@ -131,6 +131,32 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
                   const double* Ru, const double* Rh, double* dudg, double* dudg_duh,
                   double* ae, double* fe, int32_t* status);
 
+/* One request's element range in a batched elemental launch.  Device arrays
+ * laid out as for hdgx_elemental; app is a host pointer. */
+typedef struct hdgx_segment {
+  int64_t ne;
+  const double* dgnodes;
+  const int32_t* bf;
+  const double* udg;
+  const double* uh;
+  const double* sh;
+  double* ae;
+  double* fe;
+  double* dudg;
+  double* dudg_duh;
+  int32_t* status;
+  const hdgx_app* app;
+} hdgx_segment;
+
+/* Elemental operator over the element ranges of many requests in one batched
+ * launch per kernel (FP64): each segment is the hdg_elemental call of one
+ * request (kernel/hdg_elemental.m:1-203, as called per mesh partition by
+ * kernel/hdg_assemble.m:40-43), with its own physics parameters.  All segments
+ * share the master (nd, p) and the app's physics, nd, ncu, nc and localsolve;
+ * results equal those of hdgx_elemental on each segment alone. */
+int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg,
+                         const hdgx_segment* segs);
+
 /* ------------------------------------------------------------------------
  * Trace system and Newton update (kernel/hdg_solve.m:105-176 with
  * app.denseblock = 1).  The global trace unknown UH is (ncu, npf, nf): face i

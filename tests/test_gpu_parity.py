@@ -9,7 +9,7 @@ from tests.gpu_util import perturb, relerr
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
 
-from hdg_amd.problems import euler2d_square, euler3d_cube, ns2d_square  # noqa: E402
+from hdg_amd.problems import euler2d_square, euler3d_cube, ns2d_square, with_bdf1  # noqa: E402
 
 CASES = {
     "euler2d_p1": lambda: euler2d_square(3, 1, seed=21, alpha=0.3),
@@ -22,6 +22,8 @@ CASES = {
     "ns2d_p4": lambda: ns2d_square(3, 4, seed=28, av_frac=0.3),
     "euler3d_p1": lambda: euler3d_cube(2, 1, seed=29, alpha=0.3),
     "euler3d_p2": lambda: euler3d_cube(2, 2, seed=30, alpha=0.3),
+    "euler2d_p2_bdf1": lambda: with_bdf1(euler2d_square(3, 2, seed=31, alpha=0.3), 0.05, seed=3),
+    "ns2d_p3_bdf1": lambda: with_bdf1(ns2d_square(3, 3, seed=32, av_frac=0.3), 0.05, seed=4),
 }
 
 

@@ -1,0 +1,119 @@
+"""GPU tests of the batched elemental launch and the serving loop: a batch
+reproduces each request's own elemental call, a served request's result is
+the same alone and in a mix of requests, and one served BDF1 Newton step
+matches the oracle's."""
+import numpy as np
+import pytest
+import scipy.sparse.linalg as spla
+
+from tests.conftest import has_gpu
+from tests.gpu_util import relerr
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
+
+from hdg_amd.problems import euler2d_square, gather_uh, ns2d_square, with_bdf1  # noqa: E402
+from oracle import trace_oracle as TO  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hdg_amd.runtime import Context
+    return Context(0)
+
+
+@pytest.mark.parametrize("phys", ["ns", "euler"])
+def test_elemental_batch_matches_each_request_alone(ctx, phys):
+    from hdg_amd.runtime import Dev
+    print(f"[gpu] elemental batch {phys}", flush=True)
+    if phys == "ns":
+        probs = [with_bdf1(ns2d_square(n, 3, seed=50 + n, minf=m, av_frac=0.3), 0.05, seed=n)
+                 for n, m in ((2, 0.25), (3, 0.5), (4, 0.65))]
+    else:
+        probs = [euler2d_square(n, 2, seed=60 + n, minf=m, alpha=0.3) for n, m in ((2, 0.3), (5, 0.6), (3, 0.45))]
+    master = probs[0].master
+    segs, outs, singles = [], [], []
+    for P in probs:
+        npv, ncu, nc = P.master.npv, P.app.ncu, P.app.nc
+        ndf = P.master.npf * P.master.nfe
+        ne = P.ne
+        dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+               Dev.from_numpy(P.uh), Dev.from_numpy(P.sh))
+        out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+               Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+        segs.append((P.app, *dev, out))
+        outs.append(out)
+        singles.append(ctx.elemental(master, P.app, *dev))
+    ctx.elemental_batch(master, segs)
+    ctx.stream.synchronize()
+    for out, ref in zip(outs, singles):
+        for a, b in zip(out, ref):
+            assert np.array_equal(a.numpy(), b.numpy())
+
+
+def _small_workload(nreq=8, seed=7):
+    from hdg_amd.serve import RequestSpec
+    rng = np.random.default_rng(seed)
+    return [RequestSpec(i, 0.0, int(rng.integers(3, 7)), int(1 + i % 4), float(rng.uniform(0.2, 0.7)),
+                        int(rng.integers(0, 1 << 30)), 0.05, "ns", iters=1 + (i % 3 == 0)) for i in range(nreq)]
+
+
+def test_served_result_is_the_same_alone_and_in_the_mix():
+    from hdg_amd.serve import PayloadFactory, Server
+    specs = _small_workload()
+    fac = PayloadFactory()
+    print("[gpu] serve mix", flush=True)
+    srv = Server(workers=3, slots=16)
+    stats, done = srv.run(specs, factory=fac, realtime=False, keep_state=True)
+    assert stats["requests"] == len(specs)
+    mix = {s.spec.rid: (s.state.UDG.numpy(), s.state.UH.numpy()) for s in done}
+    assert stats["elemental_batches"] < sum(s.iters for s in specs)      # requests were merged
+    for rid in (0, 3, 5):
+        print(f"[gpu] serve alone {rid}", flush=True)
+        _, one = Server(workers=1, slots=1).run([specs[rid]], factory=fac, realtime=False, keep_state=True)
+        u, h = one[0].state.UDG.numpy(), one[0].state.UH.numpy()
+        assert relerr(mix[rid][0], u) <= 1e-12 and relerr(mix[rid][1], h) <= 1e-12
+
+
+def test_tight_budget_admits_one_at_a_time_with_the_same_results():
+    from hdg_amd.serve import PayloadFactory, Server
+    specs = _small_workload(5, seed=9)
+    fac = PayloadFactory()
+    print("[gpu] serve budget", flush=True)
+    _, a = Server(workers=2, slots=8).run(specs, factory=fac, realtime=False, keep_state=True)
+    _, b = Server(workers=2, slots=8, budget_bytes=1).run(specs, factory=fac, realtime=False, keep_state=True)
+    ra = {s.spec.rid: s.state.UH.numpy() for s in a}
+    rb = {s.spec.rid: s.state.UH.numpy() for s in b}
+    for k in ra:
+        assert relerr(ra[k], rb[k]) <= 1e-12
+
+
+@pytest.mark.parametrize("p", [2, 3])
+def test_served_bdf1_step_matches_the_oracle(oracle, p):
+    """One served Newton step of a BDF1 request vs the oracle: elemental on
+    the CPU, direct sparse trace solve, local update (hdg_solve.m:112-146)."""
+    from hdg_amd.serve import PayloadFactory, RequestSpec, Server
+    spec = RequestSpec(0, 0.0, 4, p, 0.45, 1234, 0.05, "ns", iters=1)
+    fac = PayloadFactory()
+    pl = fac.build(spec)
+    print(f"[gpu] served step vs oracle p={p}", flush=True)
+    _, done = Server(workers=1, slots=1, gmres_tol=1e-13).run([spec], factory=fac, realtime=False, keep_state=True)
+    u_gpu, uh_gpu = done[0].state.UDG.numpy(), done[0].state.UH.numpy()
+    m, app, mesh = pl["master"], pl["app"], pl["mesh"]
+    udg, UH = pl["udg"].copy(), pl["UH"].copy()
+    uh = gather_uh(UH, mesh.elcon)
+    AE, FE, DUDG, DUDG_DUH, st = oracle.elemental(m, app, mesh.dgnodes, mesh.bf, udg, uh, pl["sh"])
+    A, b = TO.sparse_system(AE, FE, mesh.elcon, app.ncu, mesh.nsiz)
+    DUH = spla.spsolve(A.tocsc(), b).reshape(app.ncu, -1, order="F")
+    udg[:, :, :] += TO.local_update(DUDG, DUDG_DUH, DUH, mesh.elcon)
+    UH = UH + DUH
+    assert relerr(uh_gpu.reshape(UH.shape, order="F"), UH) < 1e-9
+    assert relerr(u_gpu, udg) < 1e-9
+
+
+def test_realtime_serving_reports_latency():
+    from hdg_amd.serve import Server, make_workload
+    specs = make_workload(12, seed=3, rate=200.0, ne_lo=200, ne_hi=3000)
+    print("[gpu] realtime serve", flush=True)
+    stats, done = Server(workers=2, slots=8).run(specs)
+    assert stats["requests"] == 12
+    assert 0 < stats["p50_ms"] <= stats["p99_ms"]

@@ -12,7 +12,7 @@
 import numpy as np
 import pytest
 
-from hdg_amd.problems import euler2d_square, ns2d_square, euler3d_cube, gather_uh, inituhat
+from hdg_amd.problems import euler2d_square, ns2d_square, euler3d_cube, gather_uh, inituhat, with_bdf1
 
 
 def _mats(oracle, P, udg=None, uh=None):
@@ -47,6 +47,9 @@ CASES = {
     "ns2d_p2": lambda: ns2d_square(2, 2, seed=5, av_frac=0.5),
     "euler3d_p1": lambda: euler3d_cube(1, 1, seed=6, alpha=0.4),
     "euler3d_p2": lambda: euler3d_cube(1, 2, seed=7, alpha=0.4),
+    # BDF1 time term: s += SH - fc_u u, s_udg -= fc_u (volint.m:180-221)
+    "euler2d_p2_bdf1": lambda: with_bdf1(euler2d_square(2, 2, seed=3, alpha=0.4), 0.05, seed=1),
+    "ns2d_p2_bdf1": lambda: with_bdf1(ns2d_square(2, 2, seed=5, av_frac=0.5), 0.05, seed=2),
 }
 
 
@@ -118,6 +121,11 @@ def test_uniform_state_zero_residual(oracle, case):
     udg[:, :ncu, :] = uinf[None, :, None]
     uh = np.zeros_like(P.uh)
     uh[:, :, :] = uinf[:, None, None]
+    if P.app.tdep:
+        # steady in time as well: u_n = u, so SH - fc_u u = 0
+        sh = np.zeros_like(P.udg)
+        sh[:, :ncu, :] = P.app.fc_u * uinf[None, :, None]
+        P.sh = np.asfortranarray(sh)
     m = _mats(oracle, P, udg=udg, uh=uh)
     scale = np.abs(m["BD"]).max()
     assert np.abs(m["Ru"]).max() < 1e-12 * scale
