@@ -44,9 +44,10 @@ struct MfmaShape {
   static constexpr int RTH = (NH + 15) / 16;           // AE row tiles
   static constexpr int KB = C::NPV * C::NCU * C::ND;   // columns of B (q part of BD)
   static constexpr int KBS = (KB + 3) / 4;
-  // wavefronts per SIMD the register budget is set for: two (256 registers,
-  // accumulators in VGPRs) up to three row tiles, one beyond
-  static constexpr int WPE = RT <= 3 ? 2 : 1;
+  // wavefronts per SIMD the register budget is set for: 4 / 3 / 2 for one /
+  // two / three row tiles (accumulators in VGPRs, no spills), one beyond
+  // (accumulators in AGPRs)
+  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : (RT == 3 ? 2 : 1));
 };
 
 // state component of RHS column q >= 1 and its face-node index jf
@@ -78,6 +79,29 @@ HDGX_DEV constexpr bool bmie_needed(int q0, int s) {
   return false;
 }
 
+// bit s set: k-step s of B M^-1 C (tile ct) / of B M^-1 E (pass, RHS tile tb)
+// meets a nonzero block
+template <class C>
+struct SkipMasks {
+  uint64_t ct[MfmaShape<C>::RT];
+  uint64_t rhs[MfmaShape<C>::NPASS][MfmaShape<C>::TRB];
+};
+
+template <class C>
+HDGX_DEV constexpr SkipMasks<C> skip_masks() {
+  using S = MfmaShape<C>;
+  static_assert(S::KBS <= 64, "k-steps of B fit a 64-bit mask");
+  SkipMasks<C> m{};
+  for (int ct = 0; ct < S::RT; ++ct)
+    for (int s = 0; s < S::KBS; ++s)
+      if (bmic_needed<C>(ct, s)) m.ct[ct] |= 1ull << s;
+  for (int pp = 0; pp < S::NPASS; ++pp)
+    for (int tb = 0; tb < S::TRB; ++tb)
+      for (int s = 0; s < S::KBS; ++s)
+        if (bmie_needed<C>(16 * S::TRB * pp + 16 * tb, s)) m.rhs[pp][tb] |= 1ull << s;
+  return m;
+}
+
 // Element e of the batch A (one 64-lane wavefront).
 template <class C>
 HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e) {
@@ -97,22 +121,35 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   // strides padded so that lanes of a wavefront hit distinct LDS banks
   constexpr int RP = R + 1;                       // sPan / sZ: column t at t*RP
   constexpr int RSTR = 16 * CTP + 1;              // sRows: pivot row t at t*RSTR
-  constexpr int NCOLP = ((NCOL + 31) / 32) * 32 + 1;  // sXt: RHS column cc at cc*NCOLP
-  __shared__ double sPan[4 * RP];
-  __shared__ double sZ[4 * RP];
-  __shared__ double sRows[4 * RSTR];
+  constexpr int NCOLP = NCOL | 1;                 // sXt: RHS column cc at cc*NCOLP
+  // more than one RHS pass: the panel transforms (Z, pivot rows) of the first
+  // pass are kept and replayed on the later passes' RHS tiles, so D' is
+  // factored once (u-only: with q the LDS would not leave room for one
+  // wavefront per SIMD)
+  constexpr bool REPLAY = S::NPASS > 1 && !UQ;
+  // sXt (the solution columns of one RHS tile) is live only in the output
+  // phase; the panel buffer, the pivot rows and rhs0 share its storage
+  constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR + R;
+  __shared__ double sXt[NXT > NSCR ? NXT : NSCR];
+  double* const sPan = sXt;
+  double* const sRows = sXt + 4 * RP;
+  double* const sR0 = sRows + 4 * RSTR;
+  // the recorded Z columns: column t of panel k at (k*4 + t)*R, rows rotated
+  // by t instead of padded (R is a power of two when REPLAY)
+  static_assert(!REPLAY || (R & (R - 1)) == 0, "rotated Z needs R a power of two");
+  __shared__ double sZall[REPLAY ? S::NPAN * 4 * R : 4 * RP];
   __shared__ int sStep[R];
-  __shared__ double sXt[NCOLP * 16];
-  __shared__ double sR0[R];
   __shared__ double sMf[UQ ? NPV * NPV : 1];
   __shared__ double sMiRq[UQ ? NPV * NRQ : 1];
   __shared__ double sMiC[UQ ? NPV * NCC : 1];
   __shared__ double sMiE[UQ ? NPV * NEE : 1];
   constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
   constexpr int KSMAX = (NNMAX * NC + 3) / 4;            // k-steps of the AE product, upper bound
+  static_assert(NPV * NC < 32768, "AE column index fits int16");
+  __shared__ int sPiv[REPLAY ? S::NPAN * 4 : 1];
   __shared__ int sAeNodes[RTH * NNMAX];
   __shared__ int sAeNN[RTH];
-  __shared__ int sAeK[RTH * 4 * KSMAX];
+  __shared__ short sAeK[RTH * 4 * KSMAX];
 
   int st = 0;
   const double* BDe = A.BD + e * (int64_t)NU * NCOL;
@@ -143,7 +180,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
     }
     sAeNN[rta] = nn;
     for (int idx = 0; idx < 4 * KSMAX; ++idx)
-      sAeK[rta * 4 * KSMAX + idx] = (idx < nn * NC) ? sAeNodes[rta * NNMAX + idx % nn] + NPV * (idx / nn) : -1;
+      sAeK[rta * 4 * KSMAX + idx] = (short)((idx < nn * NC) ? sAeNodes[rta * NNMAX + idx % nn] + NPV * (idx / nn) : -1);
   }
 
   // ---- u,q prologue: M = U'U (dpotrf 'U'), M^-1 [Rq C E] (dpotrs)
@@ -210,8 +247,10 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 
   for (int pass = 0; pass < S::NPASS; ++pass) {
     const int qbase = 16 * TRB * pass;       // first RHS column of this pass
+    const bool full = (pass == 0) || !REPLAY;
     d4 acc[RT][CTP];
     // ---- D' tiles: D, identity on the padded diagonal
+    if (full)
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -240,7 +279,11 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns, A
       // fragments (columns of B) fetched a batch of k-steps ahead
       constexpr int BB = RT >= 4 ? 2 : 4;
-#pragma unroll
+      constexpr SkipMasks<C> SM = skip_masks<C>();
+      // fully unrolled up to three row tiles; with four the unrolled loop
+      // spills, so it stays rolled and tests the skip masks at run time
+      constexpr int UNR = RT >= 4 ? 1 : (S::KBS + BB - 1) / BB;
+#pragma unroll UNR
       for (int s0 = 0; s0 < S::KBS; s0 += BB) {
         double a[BB][RT];
 #pragma unroll
@@ -258,7 +301,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           const int am = kk % NPV, mm = (kk / NPV) % NCU, dm = kk / (NPV * NCU);
 #pragma unroll
           for (int ct = 0; ct < RT; ++ct) {
-            if (!bmic_needed<C>(ct, s)) continue;
+            if (!full || !((SM.ct[ct] >> s) & 1ull)) continue;
             const int col = 16 * ct + c;
             double b = 0.0;
             if (kk < S::KB && col < NU && col / NPV == mm) b = sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
@@ -267,10 +310,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           }
 #pragma unroll
           for (int tb = 0; tb < TRB; ++tb) {
-            bool any = false;
-#pragma unroll
-            for (int pp = 0; pp < S::NPASS; ++pp) any |= (pp == pass) && bmie_needed<C>(16 * TRB * pp + 16 * tb, s);
-            if (!any) continue;
+            if (!((SM.rhs[pass][tb] >> s) & 1ull)) continue;
             const int q = qbase + 16 * tb + c;
             double b = 0.0;
             if (kk < S::KB && q >= 1 && q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
@@ -282,6 +322,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
     }
 
     // ---- Gauss-Jordan with partial pivoting, panels of four columns
+    if (full) {
     if (lane < R) sStep[lane] = -1;
     bool done = false;                        // lane r < R: row r already a pivot row
     __syncthreads();
@@ -344,9 +385,16 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       }
 #pragma unroll
       for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= 1.0;
-      if (lane < R)
+      const int pk = k0 / 4;
+      if (lane < R) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) sZ[t * RP + lane] = z[t];
+        for (int t = 0; t < 4; ++t) {
+          if constexpr (REPLAY) sZall[(pk * 4 + t) * R + ((lane + t) & (R - 1))] = z[t];
+          else sZall[t * RP + lane] = z[t];
+        }
+      }
+      if (REPLAY && lane < 4)
+        sPiv[pk * 4 + lane] = pvt[0] * (lane == 0) + pvt[1] * (lane == 1) + pvt[2] * (lane == 2) + pvt[3] * (lane == 3);
       // original (pre-panel) pivot rows, columns from tile ct0 on: the tile
       // row and register holding pivot row p are uniform, so the selection is
       // a scalar branch; lane group p&3 owns the row
@@ -374,12 +422,45 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       // S += Z * S[pivot rows, :]
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
-        const double a = sZ[g * RP + 16 * rt + c];
+        const double a = REPLAY ? sZall[(pk * 4 + g) * R + ((16 * rt + c + g) & (R - 1))] : sZall[g * RP + 16 * rt + c];
 #pragma unroll
         for (int ct = 0; ct < CTP; ++ct)
           if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
       }
       __syncthreads();
+    }
+    } else {
+      // replay the recorded panel transforms on this pass's RHS tiles
+#pragma unroll 1
+      for (int pk = 0; pk < S::NPAN; ++pk) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int p = __builtin_amdgcn_readfirstlane(sPiv[pk * 4 + t]);
+          const int sel = p >> 2;
+          double tmp[TRB];
+#pragma unroll
+          for (int tb = 0; tb < TRB; ++tb) tmp[tb] = 0.0;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+              if (sel == 4 * rt + i)
+#pragma unroll
+                for (int tb = 0; tb < TRB; ++tb) tmp[tb] = acc[rt][RT + tb][i];
+          if (g == (p & 3))
+#pragma unroll
+            for (int tb = 0; tb < TRB; ++tb) sRows[t * RSTR + 16 * (RT + tb) + c] = tmp[tb];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+          const double a = sZall[(pk * 4 + g) * R + ((16 * rt + c + g) & (R - 1))];
+#pragma unroll
+          for (int tb = 0; tb < TRB; ++tb)
+            acc[rt][RT + tb] = mfma64(a, sRows[g * RSTR + 16 * (RT + tb) + c], acc[rt][RT + tb]);
+        }
+        __syncthreads();
+      }
     }
 
     // ---- per RHS tile: u rows by pivot order, q rows, outputs, AE/FE columns
