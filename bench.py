@@ -126,7 +126,8 @@ def serving_leg(nreq, seed):
     log(f"[bench] serving: p50 {stats['p50_ms']:.1f} ms, p99 {stats['p99_ms']:.1f} ms, "
         f"{stats['requests_per_s']:.1f} req/s, {stats['elements_per_s']:.3g} el/s")
     stats["workload"] = (f"{nreq} Poisson(50/s) BDF1 NS Newton steps, 2n^2 triangles with ne log-uniform "
-                         f"500..50000 (16 mesh sizes), p uniform 1..4, M uniform 0.2..0.7, GMRES(20)+BILU0 tol 1e-6")
+                         f"500..50000 (16 mesh sizes), p uniform 1..4, M uniform 0.2..0.7, dt at acoustic CFL 10, "
+                         f"GMRES(20)+BILU0 tol 1e-6")
     return stats
 
 

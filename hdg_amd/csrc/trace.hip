@@ -13,6 +13,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstddef>
 #include <vector>
 
 #include "../../include/hdgx.h"
@@ -166,10 +167,15 @@ __global__ __launch_bounds__(64) void trace_assemble_kernel(int ncu, int npf, in
   if (lane == 0 && status) status[i] = (ok ? 0 : HDGX_ST_SINGULAR_D) | (*bad ? HDGX_ST_NONFINITE : 0);
 }
 
-// ---- y = x + sum_k K_k x(f2f(k)) (gmres.m:38-42, arraymul.c:53-70)
-__global__ __launch_bounds__(256) void trace_matvec_kernel(int ncf, int nbf, int64_t nf, const double* __restrict__ K,
+// ---- y = x + sum_k K_k x(f2f(k)) (gmres.m:38-42, arraymul.c:53-70); NCF/NBF
+// fix the block shape at compile time (0: the runtime values)
+template <int NCF, int NBF>
+__global__ __launch_bounds__(256) void trace_matvec_kernel(int ncf_, int nbf_, int64_t nf, const double* __restrict__ K,
                                                            const int32_t* __restrict__ f2f,
-                                                           const double* __restrict__ x, double* __restrict__ y) {
+                                                           const double* __restrict__ x, double* __restrict__ y,
+                                                           const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int ncf = NCF ? NCF : ncf_, nbf = NBF ? NBF : nbf_;
   const int fpb = blockDim.x / ncf;
   const int r = threadIdx.x % ncf, fl = threadIdx.x / ncf;
   if (fl >= fpb) return;
@@ -178,11 +184,13 @@ __global__ __launch_bounds__(256) void trace_matvec_kernel(int ncf, int nbf, int
   const int nb1 = nbf - 1;
   const double* Ki = K + (size_t)ncf * ncf * nb1 * i;
   double acc = 0.0;
+#pragma unroll
   for (int k = 1; k < nbf; ++k) {
     const int j = f2f[k + (int64_t)nbf * i];
     if (j < 0) continue;
     const double* Kk = Ki + (size_t)ncf * ncf * (k - 1);
     const double* xj = x + (size_t)ncf * j;
+#pragma unroll
     for (int c = 0; c < ncf; ++c) acc += Kk[r + ncf * c] * xj[c];
   }
   y[r + (int64_t)ncf * i] = x[r + (int64_t)ncf * i] + acc;
@@ -276,11 +284,14 @@ __global__ __launch_bounds__(64) void bilu0_row_kernel(int ncf, int nbf, const i
 }
 
 // forward sweep over one colour: y_i = x_i - sum_{earlier k} L_ik y_k (applyBILU0.m:12-29)
-__global__ __launch_bounds__(256) void bilu0_lower_kernel(int ncf, int nbf, const int32_t* __restrict__ rows, int nrows,
+template <int NCF, int NBF>
+__global__ __launch_bounds__(256) void bilu0_lower_kernel(int ncf_, int nbf_, const int32_t* __restrict__ rows, int nrows,
                                                           const int32_t* __restrict__ f2f,
                                                           const int32_t* __restrict__ rank,
                                                           const double* __restrict__ LU, const double* __restrict__ x,
-                                                          double* __restrict__ y) {
+                                                          double* __restrict__ y, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  const int ncf = NCF ? NCF : ncf_, nbf = NBF ? NBF : nbf_;
   const int fpb = blockDim.x / ncf;
   const int r = threadIdx.x % ncf, fl = threadIdx.x / ncf;
   if (fl >= fpb) return;
@@ -290,12 +301,14 @@ __global__ __launch_bounds__(256) void bilu0_lower_kernel(int ncf, int nbf, cons
   const int n2 = ncf * ncf, rk = rank[i];
   const double* LUi = LU + (size_t)n2 * nbf * i;
   double v = x[r + (int64_t)ncf * i];
+#pragma unroll
   for (int k = 1; k < nbf; ++k) {
     const int j = f2f[k + (int64_t)nbf * i];
     if (j < 0 || rank[j] > rk) continue;
     const double* L = LUi + (size_t)n2 * k;
     const double* yj = y + (size_t)ncf * j;
     double s = 0.0;
+#pragma unroll
     for (int c = 0; c < ncf; ++c) s += L[r + ncf * c] * yj[c];
     v -= s;
   }
@@ -304,12 +317,15 @@ __global__ __launch_bounds__(256) void bilu0_lower_kernel(int ncf, int nbf, cons
 
 // backward sweep over one colour: v_i = U_ii^-1 (y_i - sum_{later k} U_ik v_k)
 // (applyBILU0.m:31-55); the rows of a face sit in one workgroup
-__global__ __launch_bounds__(256) void bilu0_upper_kernel(int ncf, int nbf, const int32_t* __restrict__ rows, int nrows,
+template <int NCF, int NBF>
+__global__ __launch_bounds__(256) void bilu0_upper_kernel(int ncf_, int nbf_, const int32_t* __restrict__ rows, int nrows,
                                                           const int32_t* __restrict__ f2f,
                                                           const int32_t* __restrict__ rank,
                                                           const double* __restrict__ LU, const double* __restrict__ y,
-                                                          double* __restrict__ v) {
+                                                          double* __restrict__ v, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double tmp[256];
+  const int ncf = NCF ? NCF : ncf_, nbf = NBF ? NBF : nbf_;
   const int fpb = blockDim.x / ncf;
   const int r = threadIdx.x % ncf, fl = threadIdx.x / ncf;
   const int64_t ri = (int64_t)blockIdx.x * fpb + fl;
@@ -320,12 +336,14 @@ __global__ __launch_bounds__(256) void bilu0_upper_kernel(int ncf, int nbf, cons
   if (active) {
     const int rk = rank[i];
     double t = y[r + (int64_t)ncf * i];
+#pragma unroll
     for (int k = 1; k < nbf; ++k) {
       const int j = f2f[k + (int64_t)nbf * i];
       if (j < 0 || rank[j] < rk) continue;
       const double* U = LUi + (size_t)n2 * k;
       const double* vj = v + (size_t)ncf * j;
       double s = 0.0;
+#pragma unroll
       for (int c = 0; c < ncf; ++c) s += U[r + ncf * c] * vj[c];
       t -= s;
     }
@@ -334,9 +352,67 @@ __global__ __launch_bounds__(256) void bilu0_upper_kernel(int ncf, int nbf, cons
   __syncthreads();
   if (active) {
     double s = 0.0;
+#pragma unroll
     for (int c = 0; c < ncf; ++c) s += LUi[r + ncf * c] * tmp[fl * ncf + c];
     v[r + (int64_t)ncf * i] = s;
   }
+}
+
+// block shapes (ncf, nbf) compiled with fixed sizes: 2D triangles p = 1..4
+// with ncu = 4, 3D tetrahedra p = 1, 2 with ncu = 5
+#define HDGX_TRACE_SHAPES(X) X(8, 5) X(12, 5) X(16, 5) X(20, 5) X(15, 7) X(30, 7)
+
+// Arnoldi state of one GMRES cycle, kept on the device so the host syncs only
+// every few steps: Givens-rotated Hessenberg H (column j at MAXM*j), the
+// rotations and the rotated right-hand side g (gmres.m:81-94)
+struct GmresDev {
+  int stop;           // set when converged, at maxit or on breakdown: later steps are no-ops
+  int nj;             // Arnoldi steps completed
+  int it;             // gmres.m's iteration count
+  int flag;           // -1 running, 0 converged, 1 maxit
+  double res;
+  double H[MAXM * MAXM];
+  double cs[MAXM], sn[MAXM], g[MAXM + 1];
+};
+
+__global__ void cycle_init_kernel(GmresDev* d, double beta, int it) {
+  if (threadIdx.x != 0) return;
+  d->stop = 0;
+  d->nj = 0;
+  d->it = it;
+  d->flag = -1;
+  d->res = beta;
+  for (int q = 0; q <= MAXM; ++q) d->g[q] = 0.0;
+  d->g[0] = beta;
+}
+
+// column j of H from the Gram-Schmidt coefficients hcol[0..j] and |w|^2 in
+// hcol[j+1]: rotate, extend the rotation, update the residual estimate and
+// test the stopping rules that gmres.m checks at the top of its next step
+__device__ void arnoldi_tail(GmresDev* d, int j, const double* hcol, double tolb, int maxit) {
+  if (d->stop) return;
+  double h[MAXM + 1];
+  for (int q = 0; q <= j; ++q) h[q] = hcol[q];
+  h[j + 1] = sqrt(hcol[j + 1]);
+  const bool breakdown = !(h[j + 1] > 0.0);
+  for (int q = 0; q < j; ++q) {
+    const double a = h[q], b = h[q + 1];
+    h[q] = d->cs[q] * a + d->sn[q] * b;
+    h[q + 1] = -d->sn[q] * a + d->cs[q] * b;
+  }
+  const double rr = hypot(h[j], h[j + 1]);
+  const double c = rr > 0.0 ? h[j] / rr : 1.0, sv = rr > 0.0 ? h[j + 1] / rr : 0.0;
+  d->cs[j] = c;
+  d->sn[j] = sv;
+  h[j] = rr;
+  d->g[j + 1] = -sv * d->g[j];
+  d->g[j] = c * d->g[j];
+  for (int q = 0; q <= j; ++q) d->H[q + MAXM * j] = h[q];
+  d->res = fabs(d->g[j + 1]);
+  d->nj = j + 1;
+  d->it += 1;
+  if (breakdown || d->res <= tolb) { d->stop = 1; d->flag = 0; }
+  else if (d->it >= maxit) { d->stop = 1; d->flag = 1; }
 }
 
 // ---- reductions: partial[q + m*b] = sum over block b's share of V(:,q) . w
@@ -346,8 +422,48 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// out[q] = sum_b partial[q + m*b] by one workgroup of 256 threads, in a fixed
+// order (so every run gives the same bits)
+__device__ void block_finish(int m, int nb, const double* partial, double* out) {
+  __shared__ double red[4][MAXM];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  double acc[MAXM];
+#pragma unroll
+  for (int q = 0; q < MAXM; ++q) acc[q] = 0.0;
+  // device-scope loads: the partial sums come from other workgroups
+  for (int b = threadIdx.x; b < nb; b += blockDim.x)
+#pragma unroll
+    for (int q = 0; q < MAXM; ++q)
+      if (q < m) acc[q] += __hip_atomic_load(partial + q + (int64_t)m * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+  for (int q = 0; q < MAXM; ++q)
+    if (q < m) {
+      const double sq = wave_sum(acc[q]);
+      if (lane == 0) red[wv][q] = sq;
+    }
+  __syncthreads();
+  if ((int)threadIdx.x < m) out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void finish_kernel(int m, int nb, const double* __restrict__ partial,
+                                                     double* __restrict__ out, const int* __restrict__ stop) {
+  if (stop && *stop) return;
+  block_finish(m, nb, partial, out);
+}
+
+// |w|^2 -> hcol[j+1], then the Givens step of Hessenberg column j
+__global__ __launch_bounds__(256) void finish_tail_kernel(int nb, const double* __restrict__ partial, double* hcol,
+                                                          GmresDev* d, int j, double tolb, int maxit) {
+  if (d->stop) return;
+  block_finish(1, nb, partial, hcol + j + 1);
+  __syncthreads();
+  if (threadIdx.x == 0) arnoldi_tail(d, j, hcol, tolb, maxit);
+}
+
 __global__ __launch_bounds__(RED_THREADS) void dots_kernel(int64_t n, int m, const double* __restrict__ V, int64_t ldv,
-                                                            const double* __restrict__ w, double* __restrict__ partial) {
+                                                            const double* __restrict__ w, double* __restrict__ partial,
+                                                            const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double red[RED_THREADS / 64][MAXM];
   double acc[MAXM];
 #pragma unroll
@@ -373,25 +489,12 @@ __global__ __launch_bounds__(RED_THREADS) void dots_kernel(int64_t n, int m, con
   }
 }
 
-// out[q] = sum_b partial[q + m*b]  (one workgroup)
-__global__ __launch_bounds__(256) void finish_kernel(int m, int nb, const double* __restrict__ partial,
-                                                     double* __restrict__ out) {
-  __shared__ double red[4][MAXM];
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (int q = 0; q < m; ++q) {
-    double s = 0.0;
-    for (int b = threadIdx.x; b < nb; b += blockDim.x) s += partial[q + (int64_t)m * b];
-    s = wave_sum(s);
-    if (lane == 0) red[wv][q] = s;
-  }
-  __syncthreads();
-  if ((int)threadIdx.x < m) out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] + red[3][threadIdx.x];
-}
-
 // w -= V(:, 0:m) h  and partial sums of w.w (classical Gram-Schmidt, gmres.m:70-71)
 __global__ __launch_bounds__(RED_THREADS) void cgs_update_kernel(int64_t n, int m, const double* __restrict__ V,
                                                                  int64_t ldv, const double* __restrict__ h,
-                                                                 double* __restrict__ w, double* __restrict__ partial) {
+                                                                 double* __restrict__ w, double* __restrict__ partial,
+                                                                 const int* __restrict__ stop) {
+  if (stop && *stop) return;
   __shared__ double hs[MAXM];
   __shared__ double red[RED_THREADS / 64];
   if ((int)threadIdx.x < m) hs[threadIdx.x] = h[threadIdx.x];
@@ -437,7 +540,8 @@ __global__ __launch_bounds__(RED_THREADS) void sub_norm_kernel(int64_t n, const 
 
 // out = x / sqrt(*nrm2)
 __global__ void scale_kernel(int64_t n, const double* __restrict__ x, const double* __restrict__ nrm2,
-                             double* __restrict__ out) {
+                             double* __restrict__ out, const int* __restrict__ stop) {
+  if (stop && *stop) return;
   const double s = 1.0 / sqrt(*nrm2);
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
     out[k] = x[k] * s;
@@ -527,7 +631,7 @@ int check_colors(hdgx_ctx* ctx, const hdgx_trace* T) {
 }
 
 int ensure_tws(hdgx_ctx* ctx, size_t bytes) {
-  if (!ctx->pinned) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->pinned, 64 * sizeof(double)));
+  if (!ctx->pinned) HIP_TRY(ctx, hipHostMalloc((void**)&ctx->pinned, 2048 * sizeof(double)));
   if (ctx->tws_bytes >= bytes) return HDGX_OK;
   if (ctx->tws) HIP_TRY(ctx, hipFree(ctx->tws));
   ctx->tws = nullptr;
@@ -537,37 +641,66 @@ int ensure_tws(hdgx_ctx* ctx, size_t bytes) {
   return HDGX_OK;
 }
 
-int launch_matvec(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double* x, double* y) {
+int launch_matvec(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double* x, double* y,
+                  const int* stop = nullptr) {
   const int ncf = T->ncu * T->npf, nbf = 2 * T->nfe - 1;
   const int fpb = 256 / ncf;
   const unsigned grid = (unsigned)((T->nf + fpb - 1) / fpb);
-  if (grid) hipLaunchKernelGGL(trace_matvec_kernel, dim3(grid), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf, T->nf, K, T->f2f, x, y);
+  if (grid) {
+    bool done = false;
+#define HDGX_MV(NC, NB)                                                                                           \
+    if (!done && ncf == NC && nbf == NB) {                                                                         \
+      hipLaunchKernelGGL((trace_matvec_kernel<NC, NB>), dim3(grid), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf,   \
+                         T->nf, K, T->f2f, x, y, stop);                                                            \
+      done = true;                                                                                                 \
+    }
+    HDGX_TRACE_SHAPES(HDGX_MV)
+#undef HDGX_MV
+    if (!done)
+      hipLaunchKernelGGL((trace_matvec_kernel<0, 0>), dim3(grid), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf, T->nf, K,
+                         T->f2f, x, y, stop);
+  }
   HIP_TRY(ctx, hipGetLastError());
   return HDGX_OK;
 }
 
-int launch_bilu0_apply(hdgx_ctx* ctx, const hdgx_trace* T, const double* LU, const double* x, double* y, double* tmp) {
+int launch_bilu0_apply(hdgx_ctx* ctx, const hdgx_trace* T, const double* LU, const double* x, double* y, double* tmp,
+                       const int* stop = nullptr) {
   const int ncf = T->ncu * T->npf, nbf = 2 * T->nfe - 1;
   const int fpb = 256 / ncf;
-  for (int c = 0; c < T->ncolors; ++c) {
+  auto sweep = [&](bool lower, int c) {
     const int n = T->color_ptr[c + 1] - T->color_ptr[c];
-    if (n == 0) continue;
-    hipLaunchKernelGGL(bilu0_lower_kernel, dim3((n + fpb - 1) / fpb), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf,
-                       T->color_rows + T->color_ptr[c], n, T->f2f, T->face_rank, LU, x, tmp);
-  }
-  for (int c = T->ncolors - 1; c >= 0; --c) {
-    const int n = T->color_ptr[c + 1] - T->color_ptr[c];
-    if (n == 0) continue;
-    hipLaunchKernelGGL(bilu0_upper_kernel, dim3((n + fpb - 1) / fpb), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf,
-                       T->color_rows + T->color_ptr[c], n, T->f2f, T->face_rank, LU, tmp, y);
-  }
+    if (n == 0) return;
+    const int32_t* rows = T->color_rows + T->color_ptr[c];
+    const dim3 grid((n + fpb - 1) / fpb), block(fpb * ncf);
+#define HDGX_SW(NC, NB)                                                                                           \
+    if (ncf == NC && nbf == NB) {                                                                                  \
+      if (lower)                                                                                                   \
+        hipLaunchKernelGGL((bilu0_lower_kernel<NC, NB>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,  \
+                           T->face_rank, LU, x, tmp, stop);                                                        \
+      else                                                                                                         \
+        hipLaunchKernelGGL((bilu0_upper_kernel<NC, NB>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,  \
+                           T->face_rank, LU, tmp, y, stop);                                                        \
+      return;                                                                                                      \
+    }
+    HDGX_TRACE_SHAPES(HDGX_SW)
+#undef HDGX_SW
+    if (lower)
+      hipLaunchKernelGGL((bilu0_lower_kernel<0, 0>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,
+                         T->face_rank, LU, x, tmp, stop);
+    else
+      hipLaunchKernelGGL((bilu0_upper_kernel<0, 0>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,
+                         T->face_rank, LU, tmp, y, stop);
+  };
+  for (int c = 0; c < T->ncolors; ++c) sweep(true, c);
+  for (int c = T->ncolors - 1; c >= 0; --c) sweep(false, c);
   HIP_TRY(ctx, hipGetLastError());
   return HDGX_OK;
 }
 
 // sum of m values reduced from the partial buffer into dev_out, copied to host
 int reduce_to_host(hdgx_ctx* ctx, int m, int nb, const double* partial, double* dev_out, double* host) {
-  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, ctx->stream, m, nb, partial, dev_out);
+  hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, ctx->stream, m, nb, partial, dev_out, (const int*)nullptr);
   HIP_TRY(ctx, hipGetLastError());
   if (host) {
     HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, dev_out, m * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -660,8 +793,9 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
   const int64_t N = (int64_t)T->ncu * T->npf * T->nf;
   if (maxit <= 0) maxit = (int)std::min<int64_t>(N, 1 << 30);
   const int m1 = restart + 1;
-  // workspace: V (N x m1) | t | u | b~ | partial | dev scalars
-  const size_t words = (size_t)N * (m1 + 3) + (size_t)RED_BLOCKS * MAXM + 2 * MAXM;
+  // workspace: V (N x m1) | t | u | b~ | partial | h | y | Arnoldi state
+  const size_t dwords = (sizeof(GmresDev) + 7) / 8;
+  const size_t words = (size_t)N * (m1 + 3) + (size_t)RED_BLOCKS * MAXM + 2 * MAXM + dwords;
   if ((rc = ensure_tws(ctx, words * sizeof(double)))) return rc;
   double* V = static_cast<double*>(ctx->tws);
   double* t = V + (size_t)N * m1;
@@ -670,15 +804,18 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
   double* partial = bt + N;
   double* hdev = partial + (size_t)RED_BLOCKS * MAXM;
   double* ydev = hdev + MAXM;
+  GmresDev* dst = reinterpret_cast<GmresDev*>(ydev + MAXM);
+  const int* stop = &dst->stop;
   const unsigned rg = grid_for(N, RED_THREADS, RED_BLOCKS);
   hipStream_t s = ctx->stream;
+  constexpr int SYNC_EVERY = 4;     // Arnoldi steps launched between host syncs
 
   // operator: out = A v (plus the ILU(0) solve when LU is given)
-  auto op = [&](const double* v, double* out) -> int {
-    if (!LU) return launch_matvec(ctx, T, K, v, out);
-    int r = launch_matvec(ctx, T, K, v, u);
+  auto op = [&](const double* v, double* out, const int* stp) -> int {
+    if (!LU) return launch_matvec(ctx, T, K, v, out, stp);
+    int r = launch_matvec(ctx, T, K, v, u, stp);
     if (r) return r;
-    return launch_bilu0_apply(ctx, T, LU, u, out, t);   // out is never t
+    return launch_bilu0_apply(ctx, T, LU, u, out, t, stp);   // out is never t
   };
 
   // b~ = P^-1 F (gmres_bilu0.m:18) or F
@@ -701,87 +838,68 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
     if (flag) *flag = 0;
     return HDGX_OK;
   }
-  double H[MAXM][MAXM], cs[MAXM], sn[MAXM], g[MAXM + 1];
+  const double tolb = tol * nrmb;
+  GmresDev* hst = reinterpret_cast<GmresDev*>(ctx->pinned);
   while (true) {
     // r = b - A x (gmres.m:37-47)
-    if (LU) {
-      if ((rc = op(x, V))) return rc;                  // V(:,0) as scratch for A x
-      hipLaunchKernelGGL(sub_norm_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, b, (const double*)V, t, partial);
-    } else {
-      if ((rc = launch_matvec(ctx, T, K, x, V))) return rc;
-      hipLaunchKernelGGL(sub_norm_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, b, (const double*)V, t, partial);
-    }
+    if ((rc = op(x, V, nullptr))) return rc;                 // V(:,0) as scratch for A x
+    hipLaunchKernelGGL(sub_norm_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, b, (const double*)V, t, partial);
     if ((rc = reduce_to_host(ctx, 1, rg, partial, hdev, h))) return rc;
     const double beta = std::sqrt(h[0]);
-    hipLaunchKernelGGL(scale_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, (const double*)t, (const double*)hdev, V);
+    hipLaunchKernelGGL(scale_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, (const double*)t, (const double*)hdev, V,
+                       (const int*)nullptr);
     res = beta;
     ++it;
-    for (int q = 0; q <= restart; ++q) g[q] = 0.0;
-    g[0] = beta;
     int nj = 0;
-    for (int j = 0; j < restart; ++j) {
-      if (res / nrmb <= tol) { fl = 0; break; }
-      if (it >= maxit) { fl = 1; break; }
-      double* vj = V + (size_t)N * j;
-      double* w = V + (size_t)N * (j + 1);
-      if ((rc = op(vj, w))) return rc;
-      // h(0:j) = V(:,0:j)' w ; w -= V h ; h(j+1) = |w|
-      hipLaunchKernelGGL(dots_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, j + 1, (const double*)V, N,
-                         (const double*)w, partial);
-      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, j + 1, (int)rg, (const double*)partial, hdev);
-      hipLaunchKernelGGL(cgs_update_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, j + 1, (const double*)V, N,
-                         (const double*)hdev, w, partial + (size_t)RED_BLOCKS * (MAXM - 1));
-      hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, 1, (int)rg,
-                         (const double*)(partial + (size_t)RED_BLOCKS * (MAXM - 1)), hdev + j + 1);
-      HIP_TRY(ctx, hipGetLastError());
-      HIP_TRY(ctx, hipMemcpyAsync(ctx->pinned, hdev, (j + 2) * sizeof(double), hipMemcpyDeviceToHost, s));
-      HIP_TRY(ctx, hipStreamSynchronize(s));
-      for (int q = 0; q <= j + 1; ++q) H[q][j] = ctx->pinned[q];
-      H[j + 1][j] = std::sqrt(H[j + 1][j]);
-      const bool breakdown = !(H[j + 1][j] > 0.0);
-      if (!breakdown)
+    if (res <= tolb) fl = 0;
+    else if (it >= maxit) fl = 1;
+    if (fl < 0) {
+      hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(64), 0, s, dst, beta, it);
+      for (int j = 0; j < restart; ++j) {
+        double* vj = V + (size_t)N * j;
+        double* w = V + (size_t)N * (j + 1);
+        if ((rc = op(vj, w, stop))) return rc;
+        // h(0:j) = V(:,0:j)' w ; w -= V h ; |w|^2 (gmres.m:80-83)
+        hipLaunchKernelGGL(dots_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, j + 1, (const double*)V, N,
+                           (const double*)w, partial, stop);
+        hipLaunchKernelGGL(finish_kernel, dim3(1), dim3(256), 0, s, j + 1, (int)rg, (const double*)partial, hdev, stop);
+        hipLaunchKernelGGL(cgs_update_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, j + 1, (const double*)V, N,
+                           (const double*)hdev, w, partial + (size_t)RED_BLOCKS * (MAXM - 1), stop);
+        hipLaunchKernelGGL(finish_tail_kernel, dim3(1), dim3(256), 0, s, (int)rg,
+                           (const double*)(partial + (size_t)RED_BLOCKS * (MAXM - 1)), hdev, dst, j, tolb, maxit);
         hipLaunchKernelGGL(scale_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, (const double*)w,
-                           (const double*)(hdev + j + 1), w);
-      // Givens rotations keep the least-squares residual |g(j+1)| (gmres.m:81-85)
-      for (int q = 0; q < j; ++q) {
-        const double a = H[q][j], c2 = H[q + 1][j];
-        H[q][j] = cs[q] * a + sn[q] * c2;
-        H[q + 1][j] = -sn[q] * a + cs[q] * c2;
+                           (const double*)(hdev + j + 1), w, stop);
+        HIP_TRY(ctx, hipGetLastError());
+        if ((j + 1) % SYNC_EVERY == 0 || j == restart - 1) {
+          HIP_TRY(ctx, hipMemcpyAsync(hst, dst, offsetof(GmresDev, H), hipMemcpyDeviceToHost, s));
+          HIP_TRY(ctx, hipStreamSynchronize(s));
+          if (hst->stop) break;
+        }
       }
-      const double a = H[j][j], c2 = H[j + 1][j];
-      const double rr = std::hypot(a, c2);
-      cs[j] = rr > 0.0 ? a / rr : 1.0;
-      sn[j] = rr > 0.0 ? c2 / rr : 0.0;
-      H[j][j] = rr;
-      H[j + 1][j] = 0.0;
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = cs[j] * g[j];
-      res = std::fabs(g[j + 1]);
-      nj = j + 1;
-      ++it;
-      if (breakdown) break;
-    }
-    // x += V(:, 0:nj) y with R y = g (gmres.m:94)
-    if (nj > 0) {
-      double y[MAXM];
-      for (int q = nj - 1; q >= 0; --q) {
-        double v = g[q];
-        for (int k = q + 1; k < nj; ++k) v -= H[q][k] * y[k];
-        y[q] = v / H[q][q];
-      }
-      for (int q = 0; q < nj; ++q) ctx->pinned[q] = y[q];
-      HIP_TRY(ctx, hipMemcpyAsync(ydev, ctx->pinned, nj * sizeof(double), hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(gemv_add_kernel, dim3(grid_for(N, 256)), dim3(256), 0, s, N, nj, (const double*)V, N,
-                         (const double*)ydev, x);
-      HIP_TRY(ctx, hipGetLastError());
+      HIP_TRY(ctx, hipMemcpyAsync(hst, dst, sizeof(GmresDev), hipMemcpyDeviceToHost, s));
       HIP_TRY(ctx, hipStreamSynchronize(s));
+      nj = hst->nj;
+      it = hst->it;
+      res = hst->res;
+      fl = hst->flag;
+      // x += V(:, 0:nj) y with R y = g (gmres.m:101)
+      if (nj > 0) {
+        double y[MAXM];
+        for (int q = nj - 1; q >= 0; --q) {
+          double v = hst->g[q];
+          for (int k = q + 1; k < nj; ++k) v -= hst->H[q + MAXM * k] * y[k];
+          y[q] = v / hst->H[q + MAXM * q];
+        }
+        double* ys = ctx->pinned + (sizeof(GmresDev) + 7) / 8;
+        for (int q = 0; q < nj; ++q) ys[q] = y[q];
+        HIP_TRY(ctx, hipMemcpyAsync(ydev, ys, nj * sizeof(double), hipMemcpyHostToDevice, s));
+        hipLaunchKernelGGL(gemv_add_kernel, dim3(grid_for(N, 256)), dim3(256), 0, s, N, nj, (const double*)V, N,
+                           (const double*)ydev, x);
+        HIP_TRY(ctx, hipGetLastError());
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+      }
     }
     if (fl >= 0) break;
-    if (nj < restart && nj > 0) {
-      // breakdown (h(j+1,j) = 0): the Krylov space holds the solution
-      fl = 0;
-      break;
-    }
   }
   if (iters) *iters = it;
   if (relres) *relres = res / nrmb;
@@ -846,7 +964,7 @@ int hdgx_norm2(hdgx_ctx* ctx, int64_t n, const double* x, double* result) {
   // the trace workspace may hold a Krylov basis; the reduction uses its tail
   double* partial = static_cast<double*>(ctx->tws) + (ctx->tws_bytes / sizeof(double) - RED_BLOCKS - 2);
   const unsigned rg = grid_for(n, RED_THREADS, RED_BLOCKS);
-  hipLaunchKernelGGL(dots_kernel, dim3(rg), dim3(RED_THREADS), 0, ctx->stream, n, 1, x, n, x, partial);
+  hipLaunchKernelGGL(dots_kernel, dim3(rg), dim3(RED_THREADS), 0, ctx->stream, n, 1, x, n, x, partial, (const int*)nullptr);
   double h[1];
   if ((rc = reduce_to_host(ctx, 1, rg, partial, partial + RED_BLOCKS, h))) return rc;
   *result = std::sqrt(h[0]);

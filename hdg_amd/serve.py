@@ -55,10 +55,12 @@ class RequestSpec:
 
 
 def make_workload(nreq=512, seed=2024, rate=50.0, ne_lo=500, ne_hi=50000, levels=16, physics=PHYS_NS,
-                  dt=0.05):
+                  cfl=10.0):
     """Poisson arrivals at `rate` per second; ne log-uniform in [ne_lo, ne_hi]
     snapped to `levels` log-spaced square meshes (2*n*n triangles), p uniform
-    in 1..4, M uniform in [0.2, 0.7], all from the seed."""
+    in 1..4, M uniform in [0.2, 0.7], all from the seed.  The BDF1 step of
+    each request is dt = cfl * h / (|u| + c) with h = 1/n, |u| = 1 and the
+    free-stream sound speed c = 1/M (an acoustic CFL number)."""
     rng = np.random.default_rng(np.random.SeedSequence(int(seed)))
     t = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
     ne = np.exp(rng.uniform(np.log(ne_lo), np.log(ne_hi), size=nreq))
@@ -67,7 +69,8 @@ def make_workload(nreq=512, seed=2024, rate=50.0, ne_lo=500, ne_hi=50000, levels
     p = rng.integers(1, 5, size=nreq)
     minf = rng.uniform(0.2, 0.7, size=nreq)
     seeds = rng.integers(0, 2 ** 31, size=nreq)
-    return [RequestSpec(i, float(t[i]), int(n[i]), int(p[i]), float(minf[i]), int(seeds[i]), dt, physics)
+    dt = cfl / n / (1.0 + 1.0 / minf)
+    return [RequestSpec(i, float(t[i]), int(n[i]), int(p[i]), float(minf[i]), int(seeds[i]), float(dt[i]), physics)
             for i in range(nreq)]
 
 
