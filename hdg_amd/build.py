@@ -26,8 +26,14 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 
 def configs():
     txt = open(os.path.join(CSRC, "hdgx_launch.h")).read()
-    body = txt[txt.index("#define HDGX_FOR_EACH_CFG"):txt.index("namespace hdgx")]
+    body = txt[txt.index("#define HDGX_FOR_EACH_CFG"):txt.index("#define HDGX_FOR_EACH_FAMILY")]
     return [tuple(int(v) for v in m) for m in re.findall(r"X\((\d+),\s*(\d+),\s*(\d+)\)", body)]
+
+
+def families():
+    txt = open(os.path.join(CSRC, "hdgx_launch.h")).read()
+    body = txt[txt.index("#define HDGX_FOR_EACH_FAMILY"):txt.index("namespace hdgx")]
+    return [tuple(int(v) for v in m) for m in re.findall(r"X\((\d+),\s*(\d+)\)", body)]
 
 
 def _sources():
@@ -56,6 +62,11 @@ def build(changed_only=False, jobs=None):
         obj = os.path.join(BUILD, f"inst_{ph}_{nd}_{p}.o")
         cmd = [HIPCC, *FLAGS, f"-DHDGX_PHYS={ph}", f"-DHDGX_ND={nd}", f"-DHDGX_P={p}", "-c",
                os.path.join(CSRC, "hdgx_inst.hip"), "-o", obj]
+        jobs_list.append((obj, cmd))
+    for ph, nd in families():
+        obj = os.path.join(BUILD, f"mixed_{ph}_{nd}.o")
+        cmd = [HIPCC, *FLAGS, f"-DHDGX_PHYS={ph}", f"-DHDGX_ND={nd}", "-c", os.path.join(CSRC, "mixed_inst.hip"),
+               "-o", obj]
         jobs_list.append((obj, cmd))
     for name in ("hdgx_api", "trace"):
         obj = os.path.join(BUILD, f"{name}.o")

@@ -5,8 +5,11 @@
 #include <cstdint>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <mutex>
+#include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hdgx.h"
@@ -152,6 +155,20 @@ int with_cfg(int phys, int nd, int p, F&& f) {
   HDGX_FOR_EACH_CFG(HDGX_CASE)
 #undef HDGX_CASE
   return -1;
+}
+
+template <class F>
+int with_family(int phys, int nd, F&& f) {
+#define HDGX_FCASE(PH, ND) if (phys == PH && nd == ND) return f(std::integral_constant<int, PH>{}, std::integral_constant<int, ND>{});
+  HDGX_FOR_EACH_FAMILY(HDGX_FCASE)
+#undef HDGX_FCASE
+  return -1;
+}
+
+size_t per_element_words(int nd, int npv, int npf, int nfe, int ncu, int nc, bool uq) {
+  const int ndf = npf * nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
+  return (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
+         (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
 }
 
 int porder_of(int nd, int npv) {
@@ -415,6 +432,259 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
     if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
     if (rc) return rc;
   }
+  return HDGX_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ launch plans
+struct hdgx_plan {
+  int mode = 0, grid = 0, phys = 0, nd = 0;
+  int64_t total = 0;
+  int64_t n[HDGX_MAXP + 1] = {};            // elements per order
+  void* dev = nullptr;                      // tables
+  void* ws = nullptr;                       // element matrices of all orders
+  SegTable order_tab[HDGX_MAXP + 1] = {};   // per-order launches
+  DevMaster tab[HDGX_MAXP + 1] = {};
+  const int32_t* perm[HDGX_MAXP + 1] = {};
+  MixedTable* mixed = nullptr;              // device copy (fused launch)
+  hipStream_t side[HDGX_MAXP + 1] = {};
+  hipEvent_t fork = nullptr, join[HDGX_MAXP + 1] = {};
+  hipStream_t cap = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+namespace {
+
+int plan_record(hdgx_ctx* ctx, hdgx_plan* P, hipStream_t s) {
+  if (P->mode == HDGX_PLAN_FUSED) {
+    return with_family(P->phys, P->nd, [&](auto ph, auto nd) {
+      HIP_TRY(ctx, (mixed_launch<decltype(ph)::value, decltype(nd)::value>(s, P->mixed, P->total, P->grid)));
+      return (int)HDGX_OK;
+    });
+  }
+  const bool conc = P->mode == HDGX_PLAN_CONCURRENT;
+  if (conc) HIP_TRY(ctx, hipEventRecord(P->fork, s));
+  for (int p = HDGX_MAXP - 1; p >= 1; --p) {
+    if (P->n[p] == 0) continue;
+    hipStream_t sp = conc ? P->side[p] : s;
+    if (conc) HIP_TRY(ctx, hipStreamWaitEvent(sp, P->fork, 0));
+    const int rc = with_cfg(P->phys, P->nd, p, [&](auto cfg) {
+      HIP_TRY(ctx, seg_launch<decltype(cfg)>(sp, P->order_tab[p], P->tab[p], P->perm[p]));
+      return (int)HDGX_OK;
+    });
+    if (rc) return rc < 0 ? fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no kernel for this order") : rc;
+    if (conc) {
+      HIP_TRY(ctx, hipEventRecord(P->join[p], sp));
+      HIP_TRY(ctx, hipStreamWaitEvent(s, P->join[p], 0));
+    }
+  }
+  return HDGX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters, const hdgx_segment* segs, int mode,
+                     int grid, hdgx_plan** out) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (!out || nseg < 1 || !masters || !segs) return fail(ctx, HDGX_ERR_ARG, "plan: arguments");
+  if (mode != HDGX_PLAN_PER_ORDER && mode != HDGX_PLAN_FUSED && mode != HDGX_PLAN_CONCURRENT)
+    return fail(ctx, HDGX_ERR_ARG, "plan: mode");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const hdgx_app* a0 = segs[0].app;
+  if (!a0 || !masters[0]) return fail(ctx, HDGX_ERR_ARG, "plan: null app or master");
+  auto P = std::make_unique<hdgx_plan>();
+  P->mode = mode;
+  P->phys = a0->physics;
+  P->nd = masters[0]->nd;
+  // order the segments by decreasing p (stable)
+  std::vector<int> order(nseg), pof(nseg);
+  std::vector<DevApp> apps(nseg);
+  int rc;
+  for (int s = 0; s < nseg; ++s) {
+    const hdgx_segment& g = segs[s];
+    if (!masters[s] || !g.app) return fail(ctx, HDGX_ERR_ARG, "plan: null app or master");
+    const hdgx_app* a = g.app;
+    if (a->physics != P->phys || masters[s]->nd != P->nd || a->ncu != a0->ncu || a->nc != a0->nc ||
+        a->localsolve != a0->localsolve)
+      return fail(ctx, HDGX_ERR_ARG, "plan: segments must share physics, nd, ncu, nc and localsolve");
+    const int p = porder_of(P->nd, masters[s]->npv);
+    if (p < 1 || p >= HDGX_MAXP) return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: order");
+    pof[s] = p;
+    order[s] = s;
+    if ((rc = make_app(ctx, masters[s], a, &apps[s]))) return rc;
+    if (g.ne < 0) return fail(ctx, HDGX_ERR_ARG, "plan: ne < 0");
+    if (g.ne > 0 && (!g.dgnodes || !g.bf || !g.udg || !g.uh || !g.ae || !g.fe || !g.dudg || !g.dudg_duh))
+      return fail(ctx, HDGX_ERR_ARG, "plan: null array");
+    if (g.ne > 0 && a->tdep && !g.sh) return fail(ctx, HDGX_ERR_ARG, "plan: tdep needs sh");
+    P->n[p] += g.ne;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pof[x] > pof[y]; });
+  int64_t first[HDGX_MAXP + 1] = {};
+  {
+    int64_t acc = 0;
+    for (int p = HDGX_MAXP - 1; p >= 1; --p) { first[p] = acc; acc += P->n[p]; }
+    P->total = acc;
+    first[HDGX_MAXP] = acc;
+    first[0] = acc;
+  }
+  // batch-global table (fused) and per-order tables (order-relative e0)
+  std::vector<ElemSeg> glob(nseg), per(nseg);
+  int64_t cur[HDGX_MAXP + 1] = {};
+  int start[HDGX_MAXP + 1] = {}, count[HDGX_MAXP + 1] = {};
+  for (int k = 0; k < nseg; ++k) {
+    const int s = order[k], p = pof[s];
+    const hdgx_segment& g = segs[s];
+    if (count[p] == 0) start[p] = k;
+    ++count[p];
+    glob[k] = ElemSeg{first[p] + cur[p], g.ne, g.dgnodes, g.bf, g.udg, g.uh, g.sh, g.ae, g.fe, g.dudg, g.dudg_duh,
+                      g.status, s, p};
+    per[k] = glob[k];
+    per[k].e0 = cur[p];
+    cur[p] += g.ne;
+  }
+  // masters
+  for (int p = 1; p < HDGX_MAXP; ++p) {
+    if (P->n[p] == 0) continue;
+    int s0 = -1;
+    for (int s = 0; s < nseg; ++s) if (pof[s] == p) { s0 = s; break; }
+    MasterBuf* mb = nullptr;
+    if ((rc = get_master(ctx, masters[s0], &mb))) return rc;
+    for (int s = 0; s < nseg; ++s)
+      if (pof[s] == p && masters[s]->npv != mb->npv) return fail(ctx, HDGX_ERR_ARG, "plan: masters of one order differ");
+    P->tab[p] = mb->tab;
+    P->perm[p] = mb->tab.perm;
+  }
+  // device tables: glob | per | apps | mixed
+  const size_t b1 = sizeof(ElemSeg) * nseg, b2 = sizeof(DevApp) * nseg, b3 = sizeof(MixedTable);
+  HIP_TRY(ctx, hipMalloc(&P->dev, 2 * b1 + b2 + b3 + 64));
+  char* d = static_cast<char*>(P->dev);
+  ElemSeg* dglob = reinterpret_cast<ElemSeg*>(d);
+  ElemSeg* dper = reinterpret_cast<ElemSeg*>(d + b1);
+  DevApp* dapps = reinterpret_cast<DevApp*>(d + 2 * b1);
+  P->mixed = reinterpret_cast<MixedTable*>(d + 2 * b1 + ((b2 + 15) / 16) * 16);
+  // workspace, per order
+  const hdgx_app* a = a0;
+  const bool uq = a->localsolve != 0;
+  size_t words = 0, off[HDGX_MAXP + 1] = {};
+  for (int p = HDGX_MAXP - 1; p >= 1; --p) {
+    if (P->n[p] == 0) continue;
+    int s0 = 0;
+    for (int s = 0; s < nseg; ++s) if (pof[s] == p) { s0 = s; break; }
+    const hdgx_master* m = masters[s0];
+    off[p] = words;
+    words += per_element_words(P->nd, m->npv, m->npf, m->nfe, a->ncu, a->nc, uq) * (size_t)P->n[p];
+  }
+  HIP_TRY(ctx, hipMalloc(&P->ws, std::max<size_t>(words, 1) * 8));
+  MixedTable mt{};
+  mt.segs = dglob;
+  mt.apps = dapps;
+  mt.nseg = nseg;
+  mt.total = P->total;
+  for (int p = 0; p <= HDGX_MAXP; ++p) {
+    mt.first[p] = first[p];
+    mt.end[p] = first[p] + (p < HDGX_MAXP ? P->n[p] : 0);
+  }
+  for (int p = HDGX_MAXP - 1; p >= 1; --p) {
+    if (P->n[p] == 0) continue;
+    int s0 = 0;
+    for (int s = 0; s < nseg; ++s) if (pof[s] == p) { s0 = s; break; }
+    const hdgx_master* m = masters[s0];
+    const int npv = m->npv, ndf = m->npf * m->nfe, nu = npv * a->ncu, nh = a->ncu * ndf, ncol = npv * a->nc;
+    const int64_t n = P->n[p];
+    double* w = static_cast<double*>(P->ws) + off[p];
+    SegTable& t = P->order_tab[p];
+    t.segs = dper + start[p];
+    t.apps = dapps;
+    t.nseg = count[p];
+    t.g0 = 0;
+    t.total = n;
+    t.BD = w; w += (size_t)nu * ncol * n;
+    t.F = w; w += (size_t)nu * nh * n;
+    t.GK = w; w += (size_t)nh * ncol * n;
+    t.H = w; w += (size_t)nh * nh * n;
+    t.Ru = w; w += (size_t)nu * n;
+    t.Rh = w; w += (size_t)nh * n;
+    if (uq) {
+      t.M = w; w += (size_t)npv * npv * n;
+      t.C = w; w += (size_t)npv * npv * P->nd * n;
+      t.E = w; w += (size_t)npv * ndf * P->nd * n;
+      t.Rq = w; w += (size_t)npv * a->ncu * P->nd * n;
+    }
+    mt.masters[p] = P->tab[p];
+    mt.BD[p] = t.BD; mt.F[p] = t.F; mt.GK[p] = t.GK; mt.H[p] = t.H; mt.Ru[p] = t.Ru; mt.Rh[p] = t.Rh;
+    mt.M[p] = t.M; mt.C[p] = t.C; mt.E[p] = t.E; mt.Rq[p] = t.Rq;
+  }
+  HIP_TRY(ctx, hipMemcpy(dglob, glob.data(), b1, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(dper, per.data(), b1, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(dapps, apps.data(), b2, hipMemcpyHostToDevice));
+  HIP_TRY(ctx, hipMemcpy(P->mixed, &mt, b3, hipMemcpyHostToDevice));
+  int ncu_dev = 256;
+  (void)hipDeviceGetAttribute(&ncu_dev, hipDeviceAttributeMultiprocessorCount, ctx->device);
+  P->grid = grid > 0 ? grid : 4 * ncu_dev;
+  if (mode == HDGX_PLAN_FUSED &&
+      with_family(P->phys, P->nd, [](auto, auto) { return 0; }) < 0)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no fused kernel for this physics and dimension");
+  if (mode == HDGX_PLAN_CONCURRENT) {
+    HIP_TRY(ctx, hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
+    for (int p = 1; p < HDGX_MAXP; ++p) {
+      if (P->n[p] == 0) continue;
+      HIP_TRY(ctx, hipStreamCreateWithFlags(&P->side[p], hipStreamNonBlocking));
+      HIP_TRY(ctx, hipEventCreateWithFlags(&P->join[p], hipEventDisableTiming));
+    }
+  }
+  *out = P.release();
+  return HDGX_OK;
+}
+
+int hdgx_plan_launch(hdgx_ctx* ctx, hdgx_plan* plan) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (!plan) return fail(ctx, HDGX_ERR_ARG, "plan: null");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (plan->exec) {
+    HIP_TRY(ctx, hipGraphLaunch(plan->exec, ctx->stream));
+    return HDGX_OK;
+  }
+  return plan_record(ctx, plan, ctx->stream);
+}
+
+int hdgx_plan_capture(hdgx_ctx* ctx, hdgx_plan* plan) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (!plan) return fail(ctx, HDGX_ERR_ARG, "plan: null");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  if (plan->exec) return HDGX_OK;
+  if (!plan->cap) HIP_TRY(ctx, hipStreamCreateWithFlags(&plan->cap, hipStreamNonBlocking));
+  HIP_TRY(ctx, hipStreamBeginCapture(plan->cap, hipStreamCaptureModeThreadLocal));
+  const int rc = plan_record(ctx, plan, plan->cap);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(plan->cap, &graph);
+  if (rc) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc;
+  }
+  if (e != hipSuccess) return fail(ctx, HDGX_ERR_DEVICE, std::string("plan capture: ") + hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(ctx, HDGX_ERR_DEVICE, std::string("plan instantiate: ") + hipGetErrorString(ei));
+  return HDGX_OK;
+}
+
+int hdgx_plan_destroy(hdgx_ctx* ctx, hdgx_plan* plan) {
+  if (!plan) return HDGX_OK;
+  if (ctx) (void)hipSetDevice(ctx->device);
+  (void)hipDeviceSynchronize();
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  for (int p = 0; p <= HDGX_MAXP; ++p) {
+    if (plan->side[p]) (void)hipStreamDestroy(plan->side[p]);
+    if (plan->join[p]) (void)hipEventDestroy(plan->join[p]);
+  }
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
+  if (plan->ws) (void)hipFree(plan->ws);
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
   return HDGX_OK;
 }
 

@@ -84,7 +84,7 @@ struct ElemSeg {
   double *ae, *fe, *dudg, *dudg_duh;
   int32_t* status;
   int32_t app;
-  int32_t pad;
+  int32_t order;                   // polynomial order p (mixed-order launches)
 };
 
 // Segment table of one launch over batch elements g0 .. g0+total-1; the
@@ -96,6 +96,21 @@ struct SegTable {
   int32_t nseg;
   int64_t g0, total;
   double *BD, *F, *GK, *H, *Ru, *Rh, *M, *C, *E, *Rq;
+};
+
+// One persistent launch over the elements of several orders p (one physics
+// and dimension): batch elements are ordered by decreasing p; order p owns
+// batch elements first[p] .. end[p]-1 and its own workspace arrays.
+constexpr int HDGX_MAXP = 5;
+struct MixedTable {
+  const ElemSeg* segs;
+  const DevApp* apps;
+  int32_t nseg;
+  int64_t total;
+  int64_t first[HDGX_MAXP + 1], end[HDGX_MAXP + 1];
+  DevMaster masters[HDGX_MAXP];
+  double *BD[HDGX_MAXP], *F[HDGX_MAXP], *GK[HDGX_MAXP], *H[HDGX_MAXP], *Ru[HDGX_MAXP], *Rh[HDGX_MAXP];
+  double *M[HDGX_MAXP], *C[HDGX_MAXP], *E[HDGX_MAXP], *Rq[HDGX_MAXP];
 };
 
 }  // namespace hdgx

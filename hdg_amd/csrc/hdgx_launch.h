@@ -13,6 +13,9 @@
   X(1, 2, 1) X(1, 2, 2) X(1, 2, 3) X(1, 2, 4) \
   X(0, 3, 1) X(0, 3, 2)
 
+// The (physics, nd) families with a mixed-order persistent launch.
+#define HDGX_FOR_EACH_FAMILY(X) X(0, 2) X(1, 2) X(0, 3)
+
 namespace hdgx {
 
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
@@ -30,5 +33,12 @@ template <class C> hipError_t seg_launch(hipStream_t s, const SegTable& t, const
                                                         const int32_t*);
 HDGX_FOR_EACH_CFG(HDGX_DECLARE)
 #undef HDGX_DECLARE
+
+// one persistent launch per kernel over the elements of every order
+template <int PH, int ND> hipError_t mixed_launch(hipStream_t s, const MixedTable* t, int64_t total, int grid);
+#define HDGX_DECLARE_FAMILY(PH, ND) \
+  extern template hipError_t mixed_launch<PH, ND>(hipStream_t, const MixedTable*, int64_t, int);
+HDGX_FOR_EACH_FAMILY(HDGX_DECLARE_FAMILY)
+#undef HDGX_DECLARE_FAMILY
 
 }  // namespace hdgx

@@ -118,9 +118,48 @@ HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const doub
   }
 }
 
-// Element e of the batch A (a workgroup of QUAD_THREADS threads).
+// LDS of one element's quadrature (one workgroup).
 template <class C>
-HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e) {
+struct QuadLds {
+  static constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
+  static constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NH = C::NH;
+  static constexpr int NR = C::NR, NFG = NFE * NGF;
+  static constexpr bool UQ = C::UQ;
+  double sX[NPV * (ND + 1)];
+  double sU[NPV * NC];
+  double sUH[NH];
+  double sSH[NPV * NC];
+  int32_t sBf[NFE];
+  int32_t sLoc[NPV * NFE];
+  double sShV[NGV * NPV];
+  double sShF[NGF * NPF];
+  double sSff[NPF * NPF * NGF];
+  double sJac[NGV];
+  double sXx[NGV * ND * ND];
+  double sAv[NGV];
+  double sUg[NGV * NC];
+  double sSg[NGV * NCU];
+  double sNl[NFG * ND];
+  double sFJ[NFG];
+  double sAvf[NFG];
+  double sUf[NFG * NC];
+  double sUHf[NFG * NCU];
+  double sWrk[NR * NCU];
+  double sWrl[NR * NCU * NC];
+  double sFH[NFG * NCU];
+  double sFHu[NFG * NCU * NC];
+  double sFHh[NFG * NCU * NCU];
+  double sGH[NFG * NCU];
+  double sGHu[NFG * NCU * NC];
+  double sGHh[NFG * NCU * NCU];
+  double sM[UQ ? NPV * NPV : 1];
+  double sC[UQ ? NPV * NPV * ND : 1];
+  double sE[UQ ? NPV * NDF * ND : 1];
+};
+
+// Element e of the batch A (a workgroup of QUAD_THREADS threads), LDS in L.
+template <class C>
+HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e, QuadLds<C>& L) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
   constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
@@ -132,24 +171,36 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   const int ncd = ap.ncd;
   const bool tdep = ap.tdep != 0;
 
-  __shared__ double sX[NPV * (ND + 1)];
-  __shared__ double sU[NPV * NC];
-  __shared__ double sUH[NH];
-  __shared__ double sSH[NPV * NC];
-  __shared__ int32_t sBf[NFE];
-  __shared__ int32_t sLoc[NPV * NFE];
-  __shared__ double sShV[NGV * NPV];          // shapvt(:, :, 1)
-  __shared__ double sShF[NGF * NPF];          // shapft(:, :, 1)
-  __shared__ double sSff[NPF * NPF * NGF];    // shapfgdotshapfc(:, :, 1)
-  __shared__ double sJac[NGV], sXx[NGV * ND * ND], sAv[NGV];
-  __shared__ double sUg[NGV * NC], sSg[NGV * NCU];
-  __shared__ double sNl[NFG * ND], sFJ[NFG], sAvf[NFG];
-  __shared__ double sUf[NFG * NC], sUHf[NFG * NCU];
-  __shared__ double sWrk[NR * NCU];
-  __shared__ double sWrl[NR * NCU * NC];
-  __shared__ double sFH[NFG * NCU], sFHu[NFG * NCU * NC], sFHh[NFG * NCU * NCU];
-  __shared__ double sGH[NFG * NCU], sGHu[NFG * NCU * NC], sGHh[NFG * NCU * NCU];
-  __shared__ double sM[UQ ? NPV * NPV : 1], sC[UQ ? NPV * NPV * ND : 1], sE[UQ ? NPV * NDF * ND : 1];
+  double* const sX = L.sX;
+  double* const sU = L.sU;
+  double* const sUH = L.sUH;
+  double* const sSH = L.sSH;
+  int32_t* const sBf = L.sBf;
+  int32_t* const sLoc = L.sLoc;
+  double* const sShV = L.sShV;
+  double* const sShF = L.sShF;
+  double* const sSff = L.sSff;
+  double* const sJac = L.sJac;
+  double* const sXx = L.sXx;
+  double* const sAv = L.sAv;
+  double* const sUg = L.sUg;
+  double* const sSg = L.sSg;
+  double* const sNl = L.sNl;
+  double* const sFJ = L.sFJ;
+  double* const sAvf = L.sAvf;
+  double* const sUf = L.sUf;
+  double* const sUHf = L.sUHf;
+  double* const sWrk = L.sWrk;
+  double* const sWrl = L.sWrl;
+  double* const sFH = L.sFH;
+  double* const sFHu = L.sFHu;
+  double* const sFHh = L.sFHh;
+  double* const sGH = L.sGH;
+  double* const sGHu = L.sGHu;
+  double* const sGHh = L.sGHh;
+  double* const sM = L.sM;
+  double* const sC = L.sC;
+  double* const sE = L.sE;
 
   // ---- 0. stage the element's inputs and the small master tables
   for (int i = tid; i < NPV * ncd; i += NT) sX[i] = A.dgnodes[e * NPV * ncd + i];
@@ -518,7 +569,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 
 template <class C>
 __global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
-  quad_body<C>(A, T, ap, blockIdx.x);
+  __shared__ QuadLds<C> lds;
+  quad_body<C>(A, T, ap, blockIdx.x, lds);
 }
 
 }  // namespace hdgx

@@ -102,10 +102,46 @@ HDGX_DEV constexpr SkipMasks<C> skip_masks() {
   return m;
 }
 
-// Element e of the batch A (one 64-lane wavefront).
+// LDS of one element's elimination.  Strides are padded so that the lanes of
+// a wavefront hit distinct LDS banks.
 template <class C>
-HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e) {
+struct SchurLds {
   using S = MfmaShape<C>;
+  static constexpr int R = S::R, NPV = C::NPV, NPF = C::NPF, NC = C::NC;
+  static constexpr bool UQ = C::UQ;
+  static constexpr int RP = R + 1;                        // sPan / sZ: column t at t*RP
+  static constexpr int RSTR = 16 * S::CTP + 1;            // sRows: pivot row t at t*RSTR
+  static constexpr int NCOLP = C::NCOL | 1;               // sXt: RHS column cc at cc*NCOLP
+  // more than one RHS pass: the panel transforms (Z, pivot rows) of the first
+  // pass are kept and replayed on the later passes' RHS tiles, so D' is
+  // factored once (u-only: with q the LDS would not leave room for one
+  // wavefront per SIMD)
+  static constexpr bool REPLAY = S::NPASS > 1 && !UQ;
+  static constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR + R;
+  static constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
+  static constexpr int KSMAX = (NNMAX * NC + 3) / 4;       // k-steps of the AE product, upper bound
+  // sXt (the solution columns of one RHS tile) is live only in the output
+  // phase; the panel buffer, the pivot rows and rhs0 share its storage
+  double sXt[NXT > NSCR ? NXT : NSCR];
+  // the recorded Z columns: column t of panel k at (k*4 + t)*R, rows rotated
+  // by t instead of padded (R is a power of two when REPLAY)
+  double sZall[REPLAY ? S::NPAN * 4 * R : 4 * RP];
+  double sMf[UQ ? NPV * NPV : 1];
+  double sMiRq[UQ ? NPV * C::NCU * C::ND : 1];
+  double sMiC[UQ ? NPV * NPV * C::ND : 1];
+  double sMiE[UQ ? NPV * C::NDF * C::ND : 1];
+  int sStep[R];
+  int sPiv[REPLAY ? S::NPAN * 4 : 1];
+  int sAeNodes[S::RTH * NNMAX];
+  int sAeNN[S::RTH];
+  short sAeK[S::RTH * 4 * KSMAX];
+};
+
+// Element e of the batch A (one 64-lane wavefront), LDS in L.
+template <class C>
+HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e, SchurLds<C>& L) {
+  using S = MfmaShape<C>;
+  using LD = SchurLds<C>;
   constexpr int NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, NCU = C::NCU, ND = C::ND;
   constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NC = C::NC;
   constexpr bool UQ = C::UQ;
@@ -113,43 +149,30 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   constexpr int NQ = UQ ? NPV * NCU * ND : 0;
   constexpr int NRQ = NCU * ND, NCC = NPV * ND, NEE = NDF * ND;
   static_assert(R <= 64, "one panel row per lane");
+  static_assert(!LD::REPLAY || (R & (R - 1)) == 0, "rotated Z needs R a power of two");
+  static_assert(NPV * NC < 32768, "AE column index fits int16");
 
   const int lane = threadIdx.x;
   const int g = lane >> 4, c = lane & 15;
   if (e >= A.ne) return;
 
-  // strides padded so that lanes of a wavefront hit distinct LDS banks
-  constexpr int RP = R + 1;                       // sPan / sZ: column t at t*RP
-  constexpr int RSTR = 16 * CTP + 1;              // sRows: pivot row t at t*RSTR
-  constexpr int NCOLP = NCOL | 1;                 // sXt: RHS column cc at cc*NCOLP
-  // more than one RHS pass: the panel transforms (Z, pivot rows) of the first
-  // pass are kept and replayed on the later passes' RHS tiles, so D' is
-  // factored once (u-only: with q the LDS would not leave room for one
-  // wavefront per SIMD)
-  constexpr bool REPLAY = S::NPASS > 1 && !UQ;
-  // sXt (the solution columns of one RHS tile) is live only in the output
-  // phase; the panel buffer, the pivot rows and rhs0 share its storage
-  constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR + R;
-  __shared__ double sXt[NXT > NSCR ? NXT : NSCR];
+  constexpr int RP = LD::RP, RSTR = LD::RSTR, NCOLP = LD::NCOLP;
+  constexpr bool REPLAY = LD::REPLAY;
+  constexpr int NNMAX = LD::NNMAX, KSMAX = LD::KSMAX;
+  double* const sXt = L.sXt;
   double* const sPan = sXt;
   double* const sRows = sXt + 4 * RP;
   double* const sR0 = sRows + 4 * RSTR;
-  // the recorded Z columns: column t of panel k at (k*4 + t)*R, rows rotated
-  // by t instead of padded (R is a power of two when REPLAY)
-  static_assert(!REPLAY || (R & (R - 1)) == 0, "rotated Z needs R a power of two");
-  __shared__ double sZall[REPLAY ? S::NPAN * 4 * R : 4 * RP];
-  __shared__ int sStep[R];
-  __shared__ double sMf[UQ ? NPV * NPV : 1];
-  __shared__ double sMiRq[UQ ? NPV * NRQ : 1];
-  __shared__ double sMiC[UQ ? NPV * NCC : 1];
-  __shared__ double sMiE[UQ ? NPV * NEE : 1];
-  constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
-  constexpr int KSMAX = (NNMAX * NC + 3) / 4;            // k-steps of the AE product, upper bound
-  static_assert(NPV * NC < 32768, "AE column index fits int16");
-  __shared__ int sPiv[REPLAY ? S::NPAN * 4 : 1];
-  __shared__ int sAeNodes[RTH * NNMAX];
-  __shared__ int sAeNN[RTH];
-  __shared__ short sAeK[RTH * 4 * KSMAX];
+  double* const sZall = L.sZall;
+  int* const sStep = L.sStep;
+  double* const sMf = L.sMf;
+  double* const sMiRq = L.sMiRq;
+  double* const sMiC = L.sMiC;
+  double* const sMiE = L.sMiE;
+  int* const sPiv = L.sPiv;
+  int* const sAeNodes = L.sAeNodes;
+  int* const sAeNN = L.sAeNN;
+  short* const sAeK = L.sAeK;
 
   int st = 0;
   const double* BDe = A.BD + e * (int64_t)NU * NCOL;
@@ -566,7 +589,8 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 // one wavefront per SIMD with AGPR accumulators)
 template <class C>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
-  schur_mfma_body<C>(A, perm, blockIdx.x);
+  __shared__ SchurLds<C> lds;
+  schur_mfma_body<C>(A, perm, blockIdx.x, lds);
 }
 
 }  // namespace hdgx

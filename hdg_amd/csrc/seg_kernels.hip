@@ -21,11 +21,11 @@ HDGX_DEV T uload_obj(const T* p) {
   return r;
 }
 
-HDGX_DEV int find_seg(const SegTable& S, int64_t g) {
-  int lo = 0, hi = S.nseg - 1;
+HDGX_DEV int find_seg(const ElemSeg* segs, int nseg, int64_t g) {
+  int lo = 0, hi = nseg - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    const int64_t e0 = *((const __attribute__((address_space(4))) int64_t*)&S.segs[mid].e0);
+    const int64_t e0 = *((const __attribute__((address_space(4))) int64_t*)&segs[mid].e0);
     if (e0 <= g) lo = mid;
     else hi = mid - 1;
   }
@@ -39,7 +39,7 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
   const int64_t w = blockIdx.x;
   if (w >= S.total) return;
   const int64_t g = S.g0 + w;
-  const ElemSeg sg = uload_obj(S.segs + find_seg(S, g));
+  const ElemSeg sg = uload_obj(S.segs + find_seg(S.segs, S.nseg, g));
   const DevApp& ap = S.apps[sg.app];
   const int64_t e = g - sg.e0;
   QuadArgs A;
@@ -64,7 +64,8 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
     A.M = A.C = A.E = A.Rq = nullptr;
   }
   A.status = nullptr;
-  quad_body<C>(A, T, ap, 0);
+  __shared__ QuadLds<C> lds;
+  quad_body<C>(A, T, ap, 0, lds);
 }
 
 template <class C>
@@ -73,7 +74,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
   const int64_t w = blockIdx.x;
   if (w >= S.total) return;
   const int64_t g = S.g0 + w;
-  const ElemSeg sg = uload_obj(S.segs + find_seg(S, g));
+  const ElemSeg sg = uload_obj(S.segs + find_seg(S.segs, S.nseg, g));
   const int64_t e = g - sg.e0;
   SchurArgs A;
   A.ne = 1;
@@ -97,7 +98,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
   A.fe = sg.fe + e * NH;
   A.status = sg.status ? sg.status + e : nullptr;
   A.prec = 0;
-  schur_mfma_body<C>(A, perm, 0);
+  __shared__ SchurLds<C> lds;
+  schur_mfma_body<C>(A, perm, 0, lds);
 }
 
 }  // namespace hdgx

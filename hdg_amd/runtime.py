@@ -48,6 +48,10 @@ def lib():
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
         L.hdgx_elemental_batch.argtypes = [vp, vp, i32, vp]
+        L.hdgx_plan_create.argtypes = [vp, i32, vp, vp, i32, i32, ctypes.POINTER(vp)]
+        L.hdgx_plan_launch.argtypes = [vp, vp]
+        L.hdgx_plan_capture.argtypes = [vp, vp]
+        L.hdgx_plan_destroy.argtypes = [vp, vp]
         L.hdgx_trace_assemble.argtypes = [vp] * 7
         L.hdgx_trace_matvec.argtypes = [vp] * 5
         L.hdgx_bilu0_factor.argtypes = [vp] * 5
@@ -63,6 +67,7 @@ def lib():
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq", "hdgx_elemental_batch",
+           "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
 
@@ -191,6 +196,31 @@ class Context:
                                          h.ptr, _p(s), ae.ptr, fe.ptr, dudg.ptr, dudg_duh.ptr, st.ptr))
         return ae, fe, dudg, dudg_duh, st
 
+    def _segments(self, segments):
+        keep = []
+        arr = (CSegment * max(1, len(segments)))()
+        for k, (app, dg, b, u, h, s, out) in enumerate(segments):
+            ca = app.to_c() if isinstance(app, App) else app
+            keep.append(ca)
+            ae, fe, dudg, dudg_duh, st = out
+            arr[k] = CSegment(dg.shape[2], dg.ptr.value, b.ptr.value, u.ptr.value, h.ptr.value,
+                              s.ptr.value if s is not None else None, ae.ptr.value, fe.ptr.value, dudg.ptr.value,
+                              dudg_duh.ptr.value, st.ptr.value if st is not None else None,
+                              ctypes.cast(ctypes.pointer(ca), ctypes.c_void_p).value)
+        return arr, keep
+
+    def plan(self, masters, segments, mode=0, grid=0):
+        """A reusable launch plan over segments of several orders (hdgx_plan_create):
+        masters[k] is the master of segments[k]; mode 0 per order, 1 fused
+        persistent, 2 per order on concurrent streams."""
+        arr, keep = self._segments(segments)
+        mcs = [self.master_c(m) for m in masters]
+        mp = (ctypes.c_void_p * len(masters))(*[ctypes.cast(ctypes.pointer(mc.c), ctypes.c_void_p).value
+                                                  for mc in mcs])
+        h = ctypes.c_void_p()
+        self._check(lib().hdgx_plan_create(self.h, len(segments), mp, arr, mode, grid, ctypes.byref(h)))
+        return Plan(self, h, keep=(arr, keep, mcs, segments))
+
     def elemental_batch(self, master, segments):
         """One batched elemental launch over many requests' element ranges.
         segments: list of (app, dgnodes, bf, udg, uh, sh, (ae, fe, dudg, dudg_duh, status)) with
@@ -251,6 +281,33 @@ class Context:
         self._check(lib().hdgx_schur_uq(self.h, nd, npv, ncu, nc, ndf, ne, M.ptr, C.ptr, E.ptr, BD.ptr, F.ptr,
                                         GK.ptr, H.ptr, Rq.ptr, Ru.ptr, Rh.ptr, *[o.ptr for o in out]))
         return out
+
+
+class Plan:
+    """Handle of an hdgx_plan: launch() enqueues the batch, capture() records
+    it into a hipGraph that later launches replay."""
+
+    def __init__(self, ctx, h, keep):
+        self.ctx, self.h, self._keep = ctx, h, keep
+        self.captured = False
+
+    def launch(self):
+        self.ctx._check(lib().hdgx_plan_launch(self.ctx.h, self.h))
+
+    def capture(self):
+        self.ctx._check(lib().hdgx_plan_capture(self.ctx.h, self.h))
+        self.captured = True
+
+    def close(self):
+        if self.h:
+            lib().hdgx_plan_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = {}

@@ -157,6 +157,27 @@ typedef struct hdgx_segment {
 int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg,
                          const hdgx_segment* segs);
 
+/* Launch plan: a batch of segments of several polynomial orders (one physics
+ * and dimension), with tables and element-matrix workspace set up once so
+ * that the same batch can be launched many times, and captured into a
+ * hipGraph.  masters[s] is the master of segment s (its order p).  Modes:
+ *   HDGX_PLAN_PER_ORDER   one batched launch per order, in sequence;
+ *   HDGX_PLAN_CONCURRENT  one batched launch per order, on concurrent streams;
+ *   HDGX_PLAN_FUSED       one persistent launch per kernel over all orders
+ *                         (grid workgroups; 0 = 4 per CU).
+ * Results are those of hdgx_elemental on each segment (FP64).  The plan
+ * owns its workspace; it has no counterpart in the reference, whose element
+ * loop runs one mesh at a time (kernel/hdg_assemble.m:40-43). */
+enum { HDGX_PLAN_PER_ORDER = 0, HDGX_PLAN_FUSED = 1, HDGX_PLAN_CONCURRENT = 2 };
+typedef struct hdgx_plan hdgx_plan;
+int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
+                     const hdgx_segment* segs, int mode, int grid, hdgx_plan** plan);
+/* enqueue the batch on the context's stream (replays the graph once captured) */
+int hdgx_plan_launch(hdgx_ctx* ctx, hdgx_plan* plan);
+/* capture the plan's launches into a hipGraph used by later hdgx_plan_launch */
+int hdgx_plan_capture(hdgx_ctx* ctx, hdgx_plan* plan);
+int hdgx_plan_destroy(hdgx_ctx* ctx, hdgx_plan* plan);
+
 /* ------------------------------------------------------------------------
  * Trace system and Newton update (kernel/hdg_solve.m:105-176 with
  * app.denseblock = 1).  The global trace unknown UH is (ncu, npf, nf): face i
