@@ -131,6 +131,69 @@ def serving_leg(nreq, seed):
     return stats
 
 
+def uneven_leg(torch, total, seed, steps):
+    """Mixed-order batch: `total` 2D Euler triangles whose orders p = 1..4 are
+    drawn 40/30/20/10% from the seed, as four jittered rectangles; launch plan
+    chosen by the autotuner among per-order, concurrent and fused persistent
+    launches (hipGraph replays); the FP32 + FP64-refinement form beside it."""
+    from hdg_amd.autotune import time_plan, tune
+    from hdg_amd.mesh import mkmesh_square
+    from hdg_amd.problems import euler2d_square
+    from hdg_amd.runtime import Context, Dev, PREC_FP32_IR
+    rng = np.random.default_rng(np.random.SeedSequence([seed, 3]))
+    counts = rng.multinomial(total, [0.4, 0.3, 0.2, 0.1])
+    t0 = time.perf_counter()
+    probs = []
+    for p, n_p in zip((1, 2, 3, 4), counts):
+        nx = max(1, int(round(np.sqrt(n_p / 2))))
+        ny = max(1, int(round(n_p / (2 * nx))))
+        P = euler2d_square(1, p, seed=seed + p, minf=0.5)
+        m = P.master
+        from hdg_amd.problems import _finish, euler_state
+        mesh = mkmesh_square(nx, ny, m, 0.1, seed + p)
+        udg = euler_state(mesh.dgnodes, 2, 1.4, 0.5, seed + 10 + p)
+        probs.append(_finish(m, P.app, mesh, udg, f"euler2d_p{p}_{nx}x{ny}"))
+    ne_tot = sum(P.ne for P in probs)
+    log(f"[bench] uneven: {ne_tot} elements ({[P.ne for P in probs]}) built in {time.perf_counter() - t0:.1f}s")
+    ctx = Context(0)
+    masters, segs = [], []
+    for P in probs:
+        npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.ndf, P.ne
+        dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+               Dev.from_numpy(P.uh), Dev.from_numpy(P.sh))
+        out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+               Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+        masters.append(P.master)
+        segs.append((P.app, *dev, out))
+    plan, timings = tune(ctx, masters, segs, reps=3, log=log)
+    ms = time_plan(ctx, plan, reps=steps)
+    fp64 = [[o.numpy() for o in seg[6][:4]] for seg in segs]
+    bad = int(sum(np.count_nonzero(seg[6][4].numpy()) for seg in segs))
+    plan.close()
+    # FP32 + refinement, one launch per order
+    ctx.set_precision(PREC_FP32_IR)
+
+    def step32():
+        for P, seg in zip(probs, segs):
+            ctx.elemental(P.master, P.app, *seg[1:6], out=seg[6])
+    ms32, _ = time_steps(torch, ctx.stream, step32, steps, 1)
+    errs = []
+    for seg, ref in zip(segs, fp64):
+        errs += [relerr(o.numpy(), r) for o, r in zip(seg[6][:4], ref)]
+    ctx.set_precision(0)
+    best = min(timings, key=timings.get)
+    res = {"workload": f"euler2d_mixed_p1-4_{ne_tot}", "elements": ne_tot,
+           "elements_per_order": {f"p{p}": int(P.ne) for p, P in zip((1, 2, 3, 4), probs)},
+           "ms_per_step": ms, "value": ne_tot / (ms * 1e-3), "unit": "elements/s", "plan": best,
+           "candidates_ms": timings, "status_nonzero": bad,
+           "fp32_ir": {"ms_per_step": ms32, "value": ne_tot / (ms32 * 1e-3),
+                       "max_rel_err_vs_fp64": max(errs)}}
+    log(f"[bench] uneven: {best} {ms:.3f} ms/step, {ne_tot / ms * 1e3:.4g} elements/s; fp32_ir {ms32:.3f} ms")
+    del segs
+    torch.cuda.empty_cache()
+    return res
+
+
 def tets_leg(torch, n, seed, steps):
     """3D Euler p=2 on 6 n^3 jittered tetrahedra (u-only, FP64): elemental
     throughput with the outputs resident in HBM."""
@@ -165,6 +228,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=96,
                     help="requests of the Poisson serving workload (0 skips it)")
     ap.add_argument("--tets", type=int, default=40, help="cube cells per side of the 3D Euler line (0 skips it)")
+    ap.add_argument("--uneven", type=int, default=200000, help="elements of the mixed-order batch (0 skips it)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: the elemental step needs no communication and every box here has "
@@ -258,6 +322,8 @@ def main():
     torch.cuda.empty_cache()
     if a.serve_requests > 0:
         line["serving"] = serving_leg(a.serve_requests, a.seed)
+    if a.uneven > 0:
+        line["uneven_batch"] = uneven_leg(torch, a.uneven, a.seed, min(a.steps, 5))
     if a.tets > 0:
         line["tets"] = tets_leg(torch, a.tets, a.seed, min(a.steps, 5))
     print(json.dumps(line), flush=True)
