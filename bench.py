@@ -56,29 +56,39 @@ def time_steps(torch, stream, fn, steps, warmup):
     return ev0.elapsed_time(ev1) / steps, wall
 
 
-def kernel_times(torch, ctx, P, dev, reps):
-    """Per-kernel durations, measured with events on the launch stream: the
-    quadrature kernel through hdgx_elemental_matrices and the Schur kernel
-    through hdgx_schur_uq on its outputs (the two launches hdgx_elemental
-    makes, separated)."""
+def kernel_times(torch, ctx, P, dev, out, reps):
+    """Per-kernel durations with events on the launch stream: the quadrature
+    kernel alone (hdgx_elemental_matrices) and the elimination kernel as
+    hdgx_elemental runs it (the step minus the quadrature)."""
     s = ctx.stream
     mats = ctx.elemental_matrices(P.master, P.app, *dev)
-    keys = ("M", "C", "E", "BD", "F", "GK", "H", "Rq", "Ru", "Rh")
-    ctx.schur_uq(*(mats[k] for k in keys))
     torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    tq = ts = 0.0
+    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    tq = tt = 0.0
     for _ in range(reps):
         e[0].record(s)
         mats = ctx.elemental_matrices(P.master, P.app, *dev)
         e[1].record(s)
-        ctx.schur_uq(*(mats[k] for k in keys))
         e[2].record(s)
+        ctx.elemental(P.master, P.app, *dev, out=out)
+        e[3].record(s)
         torch.cuda.synchronize()
         tq += e[0].elapsed_time(e[1])
-        ts += e[1].elapsed_time(e[2])
+        tt += e[2].elapsed_time(e[3])
     del mats
-    return tq / reps, ts / reps
+    return tq / reps, (tt - tq) / reps
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes (scaled by the same counters'
+    reading of a 1 GiB device copy), or None."""
+    path = os.path.join(ROOT, "profiles", "elemental_ns2d_p3_2x128x128_traffic.json")
+    if not os.path.exists(path):
+        return None
+    d = json.load(open(path))
+    k = d.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
 
 
 def cpu_baseline(P, nthreads):
@@ -290,7 +300,7 @@ def main():
         log(f"[bench] {name}: {ms:.3f} ms/step, {ne / ms * 1e3:.4g} elements/s (wall {wall:.2f}s)")
     ctx.set_precision(PREC_FP64)
 
-    tq, ts = kernel_times(torch, ctx, P, dev, reps=5)
+    tq, ts = kernel_times(torch, ctx, P, dev, out, reps=5)
     log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
 
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -306,7 +316,7 @@ def main():
     value = ne / (ms64 * 1e-3)
     # dominant kernel roofline (FP64 element Schur + quadrature are compute bound at this size)
     if ts >= tq:
-        kname, kflops, kms = "schur_kernel<NS2D p3>", W.schur_flops * ne, ts
+        kname, kflops, kms = "schur_mfma_kernel<NS2D p3>", W.schur_flops * ne, ts
     else:
         kname, kflops, kms = "quad_kernel<NS2D p3>", W.quad_flops * ne, tq
     achieved = kflops / (kms * 1e-3) / 1e12
@@ -335,7 +345,7 @@ def main():
                        "bytes_per_element": W.bytes, "flops_per_element": W.flops},
         "kernels_ms": {"quad_kernel": tq, "schur_kernel": ts},
         "roofline": {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(kname),
                      "flops_per_element": (W.schur_flops if ts >= tq else W.quad_flops)},
         "cpu_baseline": {"value": cpu_rate, "unit": "elements/s", "cores": ncpu, "kind": "port",
                          "sample": f"full workload ({ne} elements), oracle FP64 C++ restatement, "
