@@ -80,7 +80,7 @@ HDGX_DEV constexpr bool bmie_needed(int q0, int s) {
 }
 
 // bit s set: k-step s of B M^-1 C (tile ct) / of B M^-1 E (pass, RHS tile tb)
-// meets a nonzero block
+// meets a nonzero block; the first RHS tile also carries -B M^-1 Rq in column 0
 template <class C>
 struct SkipMasks {
   uint64_t ct[MfmaShape<C>::RT];
@@ -98,7 +98,7 @@ HDGX_DEV constexpr SkipMasks<C> skip_masks() {
   for (int pp = 0; pp < S::NPASS; ++pp)
     for (int tb = 0; tb < S::TRB; ++tb)
       for (int s = 0; s < S::KBS; ++s)
-        if (bmie_needed<C>(16 * S::TRB * pp + 16 * tb, s)) m.rhs[pp][tb] |= 1ull << s;
+        if (bmie_needed<C>(16 * S::TRB * pp + 16 * tb, s) || (pp == 0 && tb == 0)) m.rhs[pp][tb] |= 1ull << s;
   return m;
 }
 
@@ -117,11 +117,11 @@ struct SchurLds {
   // factored once (u-only: with q the LDS would not leave room for one
   // wavefront per SIMD)
   static constexpr bool REPLAY = S::NPASS > 1 && !UQ;
-  static constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR + R;
+  static constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR;
   static constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
   static constexpr int KSMAX = (NNMAX * NC + 3) / 4;       // k-steps of the AE product, upper bound
   // sXt (the solution columns of one RHS tile) is live only in the output
-  // phase; the panel buffer, the pivot rows and rhs0 share its storage
+  // phase; the panel buffer and the pivot rows share its storage
   double sXt[NXT > NSCR ? NXT : NSCR];
   // the recorded Z columns: column t of panel k at (k*4 + t)*R, rows rotated
   // by t instead of padded (R is a power of two when REPLAY)
@@ -162,7 +162,6 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   double* const sXt = L.sXt;
   double* const sPan = sXt;
   double* const sRows = sXt + 4 * RP;
-  double* const sR0 = sRows + 4 * RSTR;
   double* const sZall = L.sZall;
   int* const sStep = L.sStep;
   double* const sMf = L.sMf;
@@ -253,19 +252,6 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int i = 0; i < NPV; ++i) dst[i] = x[i];
     }
   }
-  // rhs0 = Ru (- B M^-1 Rq), one row per lane
-  if (lane < R) {
-    double v = 0.0;
-    if (lane < NU) {
-      v = A.Ru[e * NU + lane];
-      if constexpr (UQ) {
-        double s = 0.0;
-        for (int kk = 0; kk < S::KB; ++kk) s += BDe[lane + NU * (NU + kk)] * sMiRq[kk];
-        v -= s;
-      }
-    }
-    sR0[lane] = v;
-  }
   __syncthreads();
 
   for (int pass = 0; pass < S::NPASS; ++pass) {
@@ -295,7 +281,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
         for (int i = 0; i < 4; ++i) {
           const int row = 16 * rt + 4 * i + g, q = qbase + 16 * tb + c;
           double v = 0.0;
-          if (row < NU && q < NRHS) v = (q == 0) ? sR0[row] : -Fe[row + NU * rhs_t<C>(q)];
+          if (row < NU && q < NRHS) v = (q == 0) ? A.Ru[e * NU + row] : -Fe[row + NU * rhs_t<C>(q)];
           acc[rt][RT + tb][i] = v;
         }
     if constexpr (UQ) {
@@ -336,7 +322,11 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
             if (!((SM.rhs[pass][tb] >> s) & 1ull)) continue;
             const int q = qbase + 16 * tb + c;
             double b = 0.0;
-            if (kk < S::KB && q >= 1 && q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
+            // column 0: rhs0 = Ru - B M^-1 Rq; columns q >= 1: B M^-1 E - F
+            if (kk < S::KB) {
+              if (q == 0) b = -sMiRq[kk];
+              else if (q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
+            }
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[sb][rt], b, acc[rt][RT + tb]);
           }

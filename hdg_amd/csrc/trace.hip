@@ -430,11 +430,11 @@ __device__ void block_finish(int m, int nb, const double* partial, double* out) 
   double acc[MAXM];
 #pragma unroll
   for (int q = 0; q < MAXM; ++q) acc[q] = 0.0;
-  // device-scope loads: the partial sums come from other workgroups
+  // the partial sums come from an earlier launch on the same stream
   for (int b = threadIdx.x; b < nb; b += blockDim.x)
 #pragma unroll
     for (int q = 0; q < MAXM; ++q)
-      if (q < m) acc[q] += __hip_atomic_load(partial + q + (int64_t)m * b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      if (q < m) acc[q] += partial[q + (int64_t)m * b];
 #pragma unroll
   for (int q = 0; q < MAXM; ++q)
     if (q < m) {
