@@ -28,7 +28,7 @@ template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
   constexpr int CH = chunk_cols<C>();
   if (a.prec == 1)
-    hipLaunchKernelGGL((schur_kernel<C, CH, float>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+    hipLaunchKernelGGL((schur_mfma_ir_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
   else if (a.prec == 2)
     hipLaunchKernelGGL((schur_kernel<C, CH, double>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
   else

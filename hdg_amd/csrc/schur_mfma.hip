@@ -4,7 +4,7 @@
 // One wavefront per element.  The element system S = [D' | RHS] (D' = D or
 // D + B M^-1 C; RHS = [Ru, -F] or [Ru - B M^-1 Rq, B M^-1 E - F]) lives in
 // v_mfma_f64_16x16x4f64 accumulator tiles: lane l = 16g + c, register i of a
-// tile holds row 4i + g, column c.  Gauss-Jordan elimination with partial
+// tile holds row 4i + g, column c (4g + i in the FP32 form's tiles).  Gauss-Jordan elimination with partial
 // pivoting runs in panels of four columns: the panel is factored one row per
 // lane (wavefront argmax for the pivot), which yields the rank-4 transform
 // S <- S + Z * S[pivot rows, :]; that update is applied to every tile right
@@ -24,8 +24,45 @@
 namespace hdgx {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
 
 HDGX_DEV d4 mfma64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
+// the 16x16x4 MFMA of the elimination's element type.  Both forms take the A
+// operand as row l%16, k l/16 and B as k l/16, column l%16 (lane l = 16g + c);
+// their accumulators differ: register i holds row 4i + g (FP64) or row
+// 4g + i (FP32), column c.  row(i, g) is that row; reg / grp invert it.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef d4 V;
+  typedef unsigned long long K;   // pivot search key
+  static HDGX_DEV d4 op(double a, double b, d4 c) { return mfma64(a, b, c); }
+  static HDGX_DEV constexpr int row(int i, int g) { return 4 * i + g; }
+  static HDGX_DEV constexpr int reg(int r) { return r >> 2; }
+  static HDGX_DEV constexpr int grp(int r) { return r & 3; }
+  static HDGX_DEV K key(double a) {
+    return isnan(a) ? 0x7ff0000000000000ull : (unsigned long long)__double_as_longlong(a);
+  }
+};
+template <> struct Mfma<float> {
+  typedef f4 V;
+  typedef unsigned K;
+  static HDGX_DEV f4 op(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  static HDGX_DEV constexpr int row(int i, int g) { return 4 * g + i; }
+  static HDGX_DEV constexpr int reg(int r) { return r & 3; }
+  static HDGX_DEV constexpr int grp(int r) { return r >> 2; }
+  static HDGX_DEV K key(float a) { return isnan(a) ? 0x7f800000u : __float_as_uint(a); }
+};
+
+// most FP64 refinement sweeps of the FP32 form
+constexpr int IR_SWEEPS = 4;
+
+// maximum over the wavefront
+HDGX_DEV double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
 
 template <class C>
 struct MfmaShape {
@@ -103,8 +140,10 @@ HDGX_DEV constexpr SkipMasks<C> skip_masks() {
 }
 
 // LDS of one element's elimination.  Strides are padded so that the lanes of
-// a wavefront hit distinct LDS banks.
-template <class C>
+// a wavefront hit distinct LDS banks.  T is the element type of the
+// Gauss-Jordan elimination: double, or float for the FP32 + FP64 refinement
+// form, which keeps every panel transform for the refinement sweeps.
+template <class C, typename T = double>
 struct SchurLds {
   using S = MfmaShape<C>;
   static constexpr int R = S::R, NPV = C::NPV, NPF = C::NPF, NC = C::NC;
@@ -116,32 +155,46 @@ struct SchurLds {
   // pass are kept and replayed on the later passes' RHS tiles, so D' is
   // factored once (u-only: with q the LDS would not leave room for one
   // wavefront per SIMD)
-  static constexpr bool REPLAY = S::NPASS > 1 && !UQ;
-  static constexpr int NXT = NCOLP * 16, NSCR = 4 * RP + 4 * RSTR;
+  static constexpr bool IR = sizeof(T) == 4;
+  static constexpr bool REPLAY = S::NPASS > 1 && !UQ && !IR;
+  static constexpr bool RECORD = REPLAY || IR;            // Z of every panel kept
+  static constexpr bool ROT = RECORD && (R & (R - 1)) == 0;
+  static constexpr int ZSTR = ROT ? R : RP;
+  // column t of the Z of panel pk, row `row`: recorded panels rotate the rows
+  // by t when R is a power of two (bank spread without padding), else pad
+  static HDGX_DEV constexpr int zidx(int pk, int t, int row) {
+    return RECORD ? (pk * 4 + t) * ZSTR + (ROT ? ((row + t) & (R - 1)) : row) : t * RP + row;
+  }
+  static constexpr int NXT = NCOLP * 16, NSCR = IR ? 0 : 4 * RP + 4 * RSTR;
   static constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
   static constexpr int KSMAX = (NNMAX * NC + 3) / 4;       // k-steps of the AE product, upper bound
   // sXt (the solution columns of one RHS tile) is live only in the output
-  // phase; the panel buffer and the pivot rows share its storage
+  // phase; in FP64 the panel buffer and the pivot rows share its storage, in
+  // FP32 they have their own (the refinement replays panels while sXt holds
+  // the solution)
   double sXt[NXT > NSCR ? NXT : NSCR];
-  // the recorded Z columns: column t of panel k at (k*4 + t)*R, rows rotated
-  // by t instead of padded (R is a power of two when REPLAY)
-  double sZall[REPLAY ? S::NPAN * 4 * R : 4 * RP];
+  T sPanR[IR ? 4 * RP + 4 * RSTR : 1];
+  T sZall[RECORD ? S::NPAN * 4 * ZSTR : 4 * RP];
   double sMf[UQ ? NPV * NPV : 1];
   double sMiRq[UQ ? NPV * C::NCU * C::ND : 1];
   double sMiC[UQ ? NPV * NPV * C::ND : 1];
   double sMiE[UQ ? NPV * C::NDF * C::ND : 1];
   int sStep[R];
-  int sPiv[REPLAY ? S::NPAN * 4 : 1];
+  int sPiv[RECORD ? S::NPAN * 4 : 1];
   int sAeNodes[S::RTH * NNMAX];
   int sAeNN[S::RTH];
   short sAeK[S::RTH * 4 * KSMAX];
 };
 
 // Element e of the batch A (one 64-lane wavefront), LDS in L.
-template <class C>
-HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e, SchurLds<C>& L) {
+template <class C, typename T = double>
+HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ perm, const int64_t e,
+                              SchurLds<C, T>& L) {
   using S = MfmaShape<C>;
-  using LD = SchurLds<C>;
+  using LD = SchurLds<C, T>;
+  using MT = Mfma<T>;
+  typedef typename MT::V V;
+  typedef typename MT::K K;
   constexpr int NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, NCU = C::NCU, ND = C::ND;
   constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NC = C::NC;
   constexpr bool UQ = C::UQ;
@@ -149,7 +202,6 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   constexpr int NQ = UQ ? NPV * NCU * ND : 0;
   constexpr int NRQ = NCU * ND, NCC = NPV * ND, NEE = NDF * ND;
   static_assert(R <= 64, "one panel row per lane");
-  static_assert(!LD::REPLAY || (R & (R - 1)) == 0, "rotated Z needs R a power of two");
   static_assert(NPV * NC < 32768, "AE column index fits int16");
 
   const int lane = threadIdx.x;
@@ -157,12 +209,14 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   if (e >= A.ne) return;
 
   constexpr int RP = LD::RP, RSTR = LD::RSTR, NCOLP = LD::NCOLP;
-  constexpr bool REPLAY = LD::REPLAY;
+  constexpr bool IR = LD::IR, RECORD = LD::RECORD;
   constexpr int NNMAX = LD::NNMAX, KSMAX = LD::KSMAX;
   double* const sXt = L.sXt;
-  double* const sPan = sXt;
-  double* const sRows = sXt + 4 * RP;
-  double* const sZall = L.sZall;
+  T* sPan;
+  if constexpr (IR) sPan = L.sPanR;
+  else sPan = L.sXt;
+  T* const sRows = sPan + 4 * RP;
+  T* const sZall = L.sZall;
   int* const sStep = L.sStep;
   double* const sMf = L.sMf;
   double* const sMiRq = L.sMiRq;
@@ -256,8 +310,8 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 
   for (int pass = 0; pass < S::NPASS; ++pass) {
     const int qbase = 16 * TRB * pass;       // first RHS column of this pass
-    const bool full = (pass == 0) || !REPLAY;
-    d4 acc[RT][CTP];
+    const bool full = (pass == 0) || !RECORD;
+    V acc[RT][CTP];
     // ---- D' tiles: D, identity on the padded diagonal
     if (full)
 #pragma unroll
@@ -266,11 +320,11 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int ct = 0; ct < RT; ++ct)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int row = 16 * rt + 4 * i + g, col = 16 * ct + c;
+          const int row = 16 * rt + MT::row(i, g), col = 16 * ct + c;
           double v = 0.0;
           if (row < NU && col < NU) v = BDe[row + NU * col];
           else if (row == col) v = 1.0;
-          acc[rt][ct][i] = v;
+          acc[rt][ct][i] = (T)v;
         }
     // ---- RHS tiles: rhs0 | -F(:, trace column)
 #pragma unroll
@@ -279,10 +333,10 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int tb = 0; tb < TRB; ++tb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int row = 16 * rt + 4 * i + g, q = qbase + 16 * tb + c;
+          const int row = 16 * rt + MT::row(i, g), q = qbase + 16 * tb + c;
           double v = 0.0;
           if (row < NU && q < NRHS) v = (q == 0) ? A.Ru[e * NU + row] : -Fe[row + NU * rhs_t<C>(q)];
-          acc[rt][RT + tb][i] = v;
+          acc[rt][RT + tb][i] = (T)v;
         }
     if constexpr (UQ) {
       // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns, A
@@ -294,13 +348,13 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       constexpr int UNR = RT >= 4 ? 1 : (S::KBS + BB - 1) / BB;
 #pragma unroll UNR
       for (int s0 = 0; s0 < S::KBS; s0 += BB) {
-        double a[BB][RT];
+        T a[BB][RT];
 #pragma unroll
         for (int sb = 0; sb < BB; ++sb)
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt) {
             const int row = 16 * rt + c, ka = 4 * (s0 + sb) + g;     // A fragment: row l%16, k l/16
-            a[sb][rt] = (s0 + sb < S::KBS && row < NU && ka < S::KB) ? BDe[row + NU * (NU + ka)] : 0.0;
+            a[sb][rt] = (s0 + sb < S::KBS && row < NU && ka < S::KB) ? (T)BDe[row + NU * (NU + ka)] : (T)0;
           }
 #pragma unroll
         for (int sb = 0; sb < BB; ++sb) {
@@ -312,23 +366,23 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           for (int ct = 0; ct < RT; ++ct) {
             if (!full || !((SM.ct[ct] >> s) & 1ull)) continue;
             const int col = 16 * ct + c;
-            double b = 0.0;
-            if (kk < S::KB && col < NU && col / NPV == mm) b = sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
+            T b = 0;
+            if (kk < S::KB && col < NU && col / NPV == mm) b = (T)sMiC[am + NPV * (col % NPV) + NPV * NPV * dm];
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(a[sb][rt], b, acc[rt][ct]);
+            for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = MT::op(a[sb][rt], b, acc[rt][ct]);
           }
 #pragma unroll
           for (int tb = 0; tb < TRB; ++tb) {
             if (!((SM.rhs[pass][tb] >> s) & 1ull)) continue;
             const int q = qbase + 16 * tb + c;
-            double b = 0.0;
+            T b = 0;
             // column 0: rhs0 = Ru - B M^-1 Rq; columns q >= 1: B M^-1 E - F
             if (kk < S::KB) {
-              if (q == 0) b = -sMiRq[kk];
-              else if (q < NRHS && rhs_k<C>(q) == mm) b = sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
+              if (q == 0) b = (T)-sMiRq[kk];
+              else if (q < NRHS && rhs_k<C>(q) == mm) b = (T)sMiE[am + NPV * rhs_jf<C>(q) + NPV * NDF * dm];
             }
 #pragma unroll
-            for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[sb][rt], b, acc[rt][RT + tb]);
+            for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = MT::op(a[sb][rt], b, acc[rt][RT + tb]);
           }
         }
       }
@@ -354,35 +408,31 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) sPan[t * RP + 16 * rt + 4 * i + g] = acc[rt][ct0][i];
+            for (int i = 0; i < 4; ++i) sPan[t * RP + 16 * rt + MT::row(i, g)] = acc[rt][ct0][i];
       }
       __syncthreads();
-      double pr[4], z[4] = {0.0, 0.0, 0.0, 0.0};
+      T pr[4], z[4] = {0, 0, 0, 0};
 #pragma unroll
-      for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[t * RP + lane] : 0.0;
+      for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[t * RP + lane] : (T)0;
       int pvt[4];
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        // |pr| as an ordered 64-bit key whose low 6 bits carry 63-lane, so one
+        // |pr| as an ordered key whose low 6 bits carry 63-lane, so one
         // max-reduction finds the largest magnitude and, on ties, the lowest lane
-        unsigned long long key = 0ull;
-        if (lane < R && !done) {
-          const double a = fabs(pr[t]);
-          unsigned long long bits = isnan(a) ? 0x7ff0000000000000ull : (unsigned long long)__double_as_longlong(a);
-          key = (bits & ~63ull) | (unsigned long long)(63 - lane);
-        }
+        K key = 0;
+        if (lane < R && !done) key = (MT::key(fabs(pr[t])) & ~(K)63) | (K)(63 - lane);
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
-          const unsigned long long o = __shfl_xor(key, off, 64);
+          const K o = __shfl_xor(key, off, 64);
           key = o > key ? o : key;
         }
-        if ((key >> 6) == 0ull) st |= 2;
-        const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63ull));
+        if ((key >> 6) == 0) st |= 2;
+        const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63));
         pvt[t] = p;
-        const double pv = __shfl(pr[t], p, 64);
-        const double ip = 1.0 / pv;
-        z[t] = (lane == p) ? 1.0 : 0.0;
-        double ppr[4], pz[4];
+        const T pv = __shfl(pr[t], p, 64);
+        const T ip = (T)1 / pv;
+        z[t] = (lane == p) ? (T)1 : (T)0;
+        T ppr[4], pz[4];
 #pragma unroll
         for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = __shfl(pr[t2], p, 64); pz[t2] = __shfl(z[t2], p, 64); }
         if (lane == p) {
@@ -390,34 +440,31 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           for (int t2 = 0; t2 < 4; ++t2) { pr[t2] *= ip; z[t2] *= ip; }
           done = true;
         } else {
-          const double m = pr[t] * ip;
+          const T m = pr[t] * ip;
 #pragma unroll
           for (int t2 = 0; t2 < 4; ++t2) { pr[t2] -= m * ppr[t2]; z[t2] -= m * pz[t2]; }
         }
         if (lane == 0) sStep[p] = k0 + t;
       }
 #pragma unroll
-      for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= 1.0;
+      for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= (T)1;
       const int pk = k0 / 4;
       if (lane < R) {
 #pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          if constexpr (REPLAY) sZall[(pk * 4 + t) * R + ((lane + t) & (R - 1))] = z[t];
-          else sZall[t * RP + lane] = z[t];
-        }
+        for (int t = 0; t < 4; ++t) sZall[LD::zidx(pk, t, lane)] = z[t];
       }
-      if (REPLAY && lane < 4)
+      if (RECORD && lane < 4)
         sPiv[pk * 4 + lane] = pvt[0] * (lane == 0) + pvt[1] * (lane == 1) + pvt[2] * (lane == 2) + pvt[3] * (lane == 3);
       // original (pre-panel) pivot rows, columns from tile ct0 on: the tile
       // row and register holding pivot row p are uniform, so the selection is
-      // a scalar branch; lane group p&3 owns the row
+      // a scalar branch; lane group grp(p) owns the row
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         const int p = pvt[t];
-        const int sel = p >> 2;                     // 4*rt + i
-        double tmp[CTP];
+        const int sel = 4 * (p >> 4) + MT::reg(p & 15);     // 4*rt + i
+        T tmp[CTP];
 #pragma unroll
-        for (int ct = 0; ct < CTP; ++ct) tmp[ct] = 0.0;
+        for (int ct = 0; ct < CTP; ++ct) tmp[ct] = 0;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -426,7 +473,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 #pragma unroll
               for (int ct = 0; ct < CTP; ++ct)
                 if (ct >= ct0) tmp[ct] = acc[rt][ct][i];
-        if (g == (p & 3))
+        if (g == MT::grp(p & 15))
 #pragma unroll
           for (int ct = 0; ct < CTP; ++ct)
             if (ct >= ct0) sRows[t * RSTR + 16 * ct + c] = tmp[ct];
@@ -435,10 +482,10 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       // S += Z * S[pivot rows, :]
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt) {
-        const double a = REPLAY ? sZall[(pk * 4 + g) * R + ((16 * rt + c + g) & (R - 1))] : sZall[g * RP + 16 * rt + c];
+        const T a = sZall[LD::zidx(pk, g, 16 * rt + c)];
 #pragma unroll
         for (int ct = 0; ct < CTP; ++ct)
-          if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
+          if (ct >= ct0) acc[rt][ct] = MT::op(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
       }
       __syncthreads();
     }
@@ -449,10 +496,10 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           const int p = __builtin_amdgcn_readfirstlane(sPiv[pk * 4 + t]);
-          const int sel = p >> 2;
-          double tmp[TRB];
+          const int sel = 4 * (p >> 4) + MT::reg(p & 15);
+          T tmp[TRB];
 #pragma unroll
-          for (int tb = 0; tb < TRB; ++tb) tmp[tb] = 0.0;
+          for (int tb = 0; tb < TRB; ++tb) tmp[tb] = 0;
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -460,17 +507,17 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
               if (sel == 4 * rt + i)
 #pragma unroll
                 for (int tb = 0; tb < TRB; ++tb) tmp[tb] = acc[rt][RT + tb][i];
-          if (g == (p & 3))
+          if (g == MT::grp(p & 15))
 #pragma unroll
             for (int tb = 0; tb < TRB; ++tb) sRows[t * RSTR + 16 * (RT + tb) + c] = tmp[tb];
         }
         __syncthreads();
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
-          const double a = sZall[(pk * 4 + g) * R + ((16 * rt + c + g) & (R - 1))];
+          const T a = sZall[LD::zidx(pk, g, 16 * rt + c)];
 #pragma unroll
           for (int tb = 0; tb < TRB; ++tb)
-            acc[rt][RT + tb] = mfma64(a, sRows[g * RSTR + 16 * (RT + tb) + c], acc[rt][RT + tb]);
+            acc[rt][RT + tb] = MT::op(a, sRows[g * RSTR + 16 * (RT + tb) + c], acc[rt][RT + tb]);
         }
         __syncthreads();
       }
@@ -481,27 +528,134 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
     for (int tb = 0; tb < TRB; ++tb) {
       const int q0 = qbase + 16 * tb;
       if (q0 >= NRHS) break;
+      double xmx = 0.0;
 #pragma unroll
       for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int k = sStep[16 * rt + 4 * i + g];
-          if (k >= 0 && k < NU) sXt[k + NCOLP * c] = acc[rt][RT + tb][i];
+          const int k = sStep[16 * rt + MT::row(i, g)];
+          if (k >= 0 && k < NU) {
+            const double v = (double)acc[rt][RT + tb][i];
+            sXt[k + NCOLP * c] = v;
+            xmx = fmax(xmx, fabs(v));
+          }
         }
       __syncthreads();
-      if constexpr (UQ) {
-        // dq = MiRq + MiC du ; dlq = MiC dlu - MiE (x) I   (schur_uq.c:146-186)
-        const int q = q0 + c;
-        for (int rr = g; rr < NQ; rr += 4) {
-          const int n = rr % NPV, m = (rr / NPV) % NCU, dd = rr / (NPV * NCU);
-          double v = 0.0;
+      // q rows; in FP32 then FP64 refinement sweeps.  With the q rows of X
+      // consistent with its u rows, the residual of the reduced system
+      // D' x = rhs is that of the u equations of the element system,
+      // r = [Ru | -F] - BD X, formed here in FP64 from BD and the FP64
+      // solution; the correction D'^-1 r replays the recorded FP32 panel
+      // transforms.  The sweeps stop once the predicted error |d|^2 / |d_prev|
+      // (the last contraction applied once more) is below 1e-13 |x|.
+      double dprev = IR ? wave_max(xmx) : 0.0;
+      bool conv = false;
+#pragma unroll 1
+      for (int sweep = 0;; ++sweep) {
+        if constexpr (UQ) {
+          // dq = MiRq + MiC du ; dlq = MiC dlu - MiE (x) I   (schur_uq.c:146-186)
+          const int q = q0 + c;
+          for (int rr = g; rr < NQ; rr += 4) {
+            const int n = rr % NPV, m = (rr / NPV) % NCU, dd = rr / (NPV * NCU);
+            double v = 0.0;
 #pragma unroll
-          for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOLP * c];
-          if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
-          else if (q < NRHS && rhs_k<C>(q) == m) v -= sMiE[n + NPV * rhs_jf<C>(q) + NPV * NDF * dd];
-          sXt[NU + rr + NCOLP * c] = v;
+            for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOLP * c];
+            if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
+            else if (q < NRHS && rhs_k<C>(q) == m) v -= sMiE[n + NPV * rhs_jf<C>(q) + NPV * NDF * dd];
+            sXt[NU + rr + NCOLP * c] = v;
+          }
+          __syncthreads();
         }
-        __syncthreads();
+        if constexpr (!IR) {
+          break;
+        } else {
+          if (conv) break;
+          if (sweep == IR_SWEEPS) { st |= 8; break; }
+          const int q = q0 + c;
+          d4 r[RT];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int row = 16 * rt + 4 * i + g;
+              double v = 0.0;
+              if (row < NU && q < NRHS) v = (q == 0) ? A.Ru[e * NU + row] : -Fe[row + NU * rhs_t<C>(q)];
+              r[rt][i] = v;
+            }
+          constexpr int KS = (NCOL + 3) / 4, BR = 4;
+#pragma unroll 1
+          for (int s0 = 0; s0 < KS; s0 += BR) {
+            double a[BR][RT], b[BR];
+#pragma unroll
+            for (int sb = 0; sb < BR; ++sb) {
+              const int k = 4 * (s0 + sb) + g;
+              const bool in = s0 + sb < KS && k < NCOL;
+              b[sb] = in ? -sXt[k + NCOLP * c] : 0.0;
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) {
+                const int row = 16 * rt + c;
+                a[sb][rt] = (in && row < NU) ? BDe[row + NU * k] : 0.0;
+              }
+            }
+#pragma unroll
+            for (int sb = 0; sb < BR; ++sb)
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt) r[rt] = mfma64(a[sb][rt], b[sb], r[rt]);
+          }
+          // FP64 layout -> FP32 layout: row 4g + i of the FP32 tile is
+          // register g of lane 16i + c in the FP64 tile
+          V rr[RT];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              T v = 0;
+#pragma unroll
+              for (int j = 0; j < 4; ++j) {
+                const double w = __shfl(r[rt][j], 16 * i + c, 64);
+                if (g == j) v = (T)w;
+              }
+              rr[rt][i] = v;
+            }
+#pragma unroll 1
+          for (int pk = 0; pk < S::NPAN; ++pk) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+              const int p = __builtin_amdgcn_readfirstlane(sPiv[pk * 4 + t]);
+              const int sel = 4 * (p >> 4) + MT::reg(p & 15);
+              T tmp = 0;
+#pragma unroll
+              for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+                  if (sel == 4 * rt + i) tmp = rr[rt][i];
+              if (g == MT::grp(p & 15)) sRows[t * 17 + c] = tmp;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) rr[rt] = MT::op(sZall[LD::zidx(pk, g, 16 * rt + c)], sRows[g * 17 + c], rr[rt]);
+            __syncthreads();
+          }
+          double dmx = 0.0, xm = 0.0;
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int k = sStep[16 * rt + MT::row(i, g)];
+              if (k >= 0 && k < NU) {
+                const double d = (double)rr[rt][i];
+                const double v = sXt[k + NCOLP * c] + d;
+                sXt[k + NCOLP * c] = v;
+                dmx = fmax(dmx, fabs(d));
+                xm = fmax(xm, fabs(v));
+              }
+            }
+          __syncthreads();
+          dmx = wave_max(dmx);
+          xm = wave_max(xm);
+          conv = dmx == 0.0 || dmx * dmx <= 1e-13 * xm * dprev;
+          dprev = dmx;
+        }
       }
       // DUDG / DUDG_DUH columns of this tile (coalesced over rows)
       for (int cc = 0; cc < 16; ++cc) {
@@ -581,6 +735,13 @@ template <class C>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
   __shared__ SchurLds<C> lds;
   schur_mfma_body<C>(A, perm, blockIdx.x, lds);
+}
+
+// FP32 MFMA Gauss-Jordan with FP64 refinement of every solution column
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_ir_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+  __shared__ SchurLds<C, float> lds;
+  schur_mfma_body<C, float>(A, perm, blockIdx.x, lds);
 }
 
 }  // namespace hdgx
