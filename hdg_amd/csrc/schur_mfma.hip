@@ -56,6 +56,8 @@ template <> struct Mfma<float> {
 
 // most FP64 refinement sweeps of the FP32 form
 constexpr int IR_SWEEPS = 4;
+// no column: entry of the AE k-step table past an AE row tile's node set
+constexpr int AE_NONE = 255;
 
 // maximum over the wavefront
 HDGX_DEV double wave_max(double v) {
@@ -85,6 +87,9 @@ struct MfmaShape {
   // two / three row tiles (accumulators in VGPRs, no spills), one beyond
   // (accumulators in AGPRs)
   static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : (RT == 3 ? 2 : 1));
+  // the FP32 + refinement form: its LDS (every panel transform kept) allows
+  // about two wavefronts per SIMD from two row tiles on
+  static constexpr int WPE_IR = RT == 1 ? 4 : (RT <= 3 ? 2 : 1);
 };
 
 // state component of RHS column q >= 1 and its face-node index jf
@@ -165,17 +170,20 @@ struct SchurLds {
   static HDGX_DEV constexpr int zidx(int pk, int t, int row) {
     return RECORD ? (pk * 4 + t) * ZSTR + (ROT ? ((row + t) & (R - 1)) : row) : t * RP + row;
   }
-  static constexpr int NXT = NCOLP * 16, NSCR = IR ? 0 : 4 * RP + 4 * RSTR;
+  // scratch of the elimination phase inside sXt: panel, pivot rows and (not
+  // recorded) the panel's Z in FP64; the prologue's Cholesky factor of M
+  static constexpr int NXT = NCOLP * 16, NSCR = IR ? 0 : 4 * RP + 4 * RSTR + (RECORD ? 0 : 4 * RP);
+  static_assert(!UQ || NPV * NPV <= NXT, "the factor of M fits the solution tile buffer");
   static constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
   static constexpr int KSMAX = (NNMAX * NC + 3) / 4;       // k-steps of the AE product, upper bound
   // sXt (the solution columns of one RHS tile) is live only in the output
-  // phase; in FP64 the panel buffer and the pivot rows share its storage, in
-  // FP32 they have their own (the refinement replays panels while sXt holds
+  // phase; the factor of M (prologue) and, in FP64, the panel buffer, pivot
+  // rows and unrecorded Z share its storage.  In FP32 the panel buffer and
+  // pivot rows have their own (the refinement replays panels while sXt holds
   // the solution)
   double sXt[NXT > NSCR ? NXT : NSCR];
   T sPanR[IR ? 4 * RP + 4 * RSTR : 1];
-  T sZall[RECORD ? S::NPAN * 4 * ZSTR : 4 * RP];
-  double sMf[UQ ? NPV * NPV : 1];
+  T sZall[RECORD ? S::NPAN * 4 * ZSTR : 1];
   double sMiRq[UQ ? NPV * C::NCU * C::ND : 1];
   double sMiC[UQ ? NPV * NPV * C::ND : 1];
   double sMiE[UQ ? NPV * C::NDF * C::ND : 1];
@@ -183,8 +191,39 @@ struct SchurLds {
   int sPiv[RECORD ? S::NPAN * 4 : 1];
   int sAeNodes[S::RTH * NNMAX];
   int sAeNN[S::RTH];
-  short sAeK[S::RTH * 4 * KSMAX];
+  unsigned char sAeK[S::RTH * 4 * KSMAX];                // AE_NONE past the node set
 };
+
+// refinement residual of the FP32 form: A fragments of BD for k-steps
+// s0 .. s0+BR-1 (row l%16 of each row tile, column 4s + l/16; zero past NCOL)
+template <class C, int BR>
+HDGX_DEV void ir_bd_frags(const double* __restrict__ BDe, int s0, int g, int c,
+                          double (&a)[BR][MfmaShape<C>::RT]) {
+  constexpr int NU = C::NU, NCOL = C::NCOL, RT = MfmaShape<C>::RT;
+#pragma unroll
+  for (int sb = 0; sb < BR; ++sb) {
+    const int k = 4 * (s0 + sb) + g;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+      const int row = 16 * rt + c;
+      a[sb][rt] = (k < NCOL && row < NU) ? BDe[row + NU * k] : 0.0;
+    }
+  }
+}
+
+// r -= BD(:, k-steps s0..s0+BR-1) X(those rows, :), X staged in LDS
+template <class C, int BR>
+HDGX_DEV void ir_residual_batch(const double* sXt, int s0, int g, int c, const double (&a)[BR][MfmaShape<C>::RT],
+                                d4 (&r)[MfmaShape<C>::RT]) {
+  constexpr int NCOL = C::NCOL, NCOLP = C::NCOL | 1, RT = MfmaShape<C>::RT;
+#pragma unroll
+  for (int sb = 0; sb < BR; ++sb) {
+    const int k = 4 * (s0 + sb) + g;
+    const double b = (k < NCOL) ? -sXt[k + NCOLP * c] : 0.0;
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) r[rt] = mfma64(a[sb][rt], b, r[rt]);
+  }
+}
 
 // Element e of the batch A (one 64-lane wavefront), LDS in L.
 template <class C, typename T = double>
@@ -202,7 +241,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   constexpr int NQ = UQ ? NPV * NCU * ND : 0;
   constexpr int NRQ = NCU * ND, NCC = NPV * ND, NEE = NDF * ND;
   static_assert(R <= 64, "one panel row per lane");
-  static_assert(NPV * NC < 32768, "AE column index fits int16");
+  static_assert(NPV * NC < AE_NONE, "AE column index fits a byte");
 
   const int lane = threadIdx.x;
   const int g = lane >> 4, c = lane & 15;
@@ -216,16 +255,16 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   if constexpr (IR) sPan = L.sPanR;
   else sPan = L.sXt;
   T* const sRows = sPan + 4 * RP;
-  T* const sZall = L.sZall;
+  T* const sZall = RECORD ? L.sZall : sRows + 4 * RSTR;
   int* const sStep = L.sStep;
-  double* const sMf = L.sMf;
+  double* const sMf = L.sXt;
   double* const sMiRq = L.sMiRq;
   double* const sMiC = L.sMiC;
   double* const sMiE = L.sMiE;
   int* const sPiv = L.sPiv;
   int* const sAeNodes = L.sAeNodes;
   int* const sAeNN = L.sAeNN;
-  short* const sAeK = L.sAeK;
+  unsigned char* const sAeK = L.sAeK;
 
   int st = 0;
   const double* BDe = A.BD + e * (int64_t)NU * NCOL;
@@ -256,7 +295,8 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
     }
     sAeNN[rta] = nn;
     for (int idx = 0; idx < 4 * KSMAX; ++idx)
-      sAeK[rta * 4 * KSMAX + idx] = (short)((idx < nn * NC) ? sAeNodes[rta * NNMAX + idx % nn] + NPV * (idx / nn) : -1);
+      sAeK[rta * 4 * KSMAX + idx] =
+          (unsigned char)((idx < nn * NC) ? sAeNodes[rta * NNMAX + idx % nn] + NPV * (idx / nn) : AE_NONE);
   }
 
   // ---- u,q prologue: M = U'U (dpotrf 'U'), M^-1 [Rq C E] (dpotrs)
@@ -582,25 +622,17 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
               if (row < NU && q < NRHS) v = (q == 0) ? A.Ru[e * NU + row] : -Fe[row + NU * rhs_t<C>(q)];
               r[rt][i] = v;
             }
+          // k-steps over the columns of BD in batches of BR, the A fragments
+          // of the next batch loaded while this one multiplies
           constexpr int KS = (NCOL + 3) / 4, BR = 4;
+          double a0[BR][RT], a1[BR][RT];
+          ir_bd_frags<C, BR>(BDe, 0, g, c, a0);
 #pragma unroll 1
-          for (int s0 = 0; s0 < KS; s0 += BR) {
-            double a[BR][RT], b[BR];
-#pragma unroll
-            for (int sb = 0; sb < BR; ++sb) {
-              const int k = 4 * (s0 + sb) + g;
-              const bool in = s0 + sb < KS && k < NCOL;
-              b[sb] = in ? -sXt[k + NCOLP * c] : 0.0;
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt) {
-                const int row = 16 * rt + c;
-                a[sb][rt] = (in && row < NU) ? BDe[row + NU * k] : 0.0;
-              }
-            }
-#pragma unroll
-            for (int sb = 0; sb < BR; ++sb)
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt) r[rt] = mfma64(a[sb][rt], b[sb], r[rt]);
+          for (int s0 = 0; s0 < KS; s0 += 2 * BR) {
+            ir_bd_frags<C, BR>(BDe, s0 + BR, g, c, a1);
+            ir_residual_batch<C, BR>(sXt, s0, g, c, a0, r);
+            ir_bd_frags<C, BR>(BDe, s0 + 2 * BR, g, c, a0);
+            ir_residual_batch<C, BR>(sXt, s0 + BR, g, c, a1, r);
           }
           // FP64 layout -> FP32 layout: row 4g + i of the FP32 tile is
           // register g of lane 16i + c in the FP64 tile
@@ -694,7 +726,8 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
               a[sb][rta] = 0.0;
               b[sb][rta] = 0.0;
               if (s0 + sb < KSMAX) {
-                const int K = sAeK[rta * 4 * KSMAX + 4 * (s0 + sb) + g];
+                int K = sAeK[rta * 4 * KSMAX + 4 * (s0 + sb) + g];
+                if (K == AE_NONE) K = -1;
                 const int row_a = 16 * rta + c;
                 if (K >= 0) {
                   if (row_a < NH) a[sb][rta] = GKe[row_a + NH * K];
@@ -739,7 +772,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
 
 // FP32 MFMA Gauss-Jordan with FP64 refinement of every solution column
 template <class C>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_mfma_ir_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE_IR, MfmaShape<C>::WPE_IR))) void schur_mfma_ir_kernel(SchurArgs A, const int32_t* __restrict__ perm) {
   __shared__ SchurLds<C, float> lds;
   schur_mfma_body<C, float>(A, perm, blockIdx.x, lds);
 }
