@@ -147,22 +147,24 @@ HDGX_DEV constexpr SkipMasks<C> skip_masks() {
 // LDS of one element's elimination.  Strides are padded so that the lanes of
 // a wavefront hit distinct LDS banks.  T is the element type of the
 // Gauss-Jordan elimination: double, or float for the FP32 + FP64 refinement
-// form, which keeps every panel transform for the refinement sweeps.
+// form, which also eliminates identity tiles to obtain D'^-1.
 template <class C, typename T = double>
 struct SchurLds {
   using S = MfmaShape<C>;
   static constexpr int R = S::R, NPV = C::NPV, NPF = C::NPF, NC = C::NC;
   static constexpr bool UQ = C::UQ;
   static constexpr int RP = R + 1;                        // sPan / sZ: column t at t*RP
-  static constexpr int RSTR = 16 * S::CTP + 1;            // sRows: pivot row t at t*RSTR
+  static constexpr bool IR = sizeof(T) == 4;
+  // tiles per elimination: D', the pass's RHS and, in FP32, the identity
+  static constexpr int CTPX = S::CTP + (IR ? S::RT : 0);
+  static constexpr int RSTR = 16 * CTPX + 1;              // sRows: pivot row t at t*RSTR
   static constexpr int NCOLP = C::NCOL | 1;               // sXt: RHS column cc at cc*NCOLP
   // more than one RHS pass: the panel transforms (Z, pivot rows) of the first
   // pass are kept and replayed on the later passes' RHS tiles, so D' is
   // factored once (u-only: with q the LDS would not leave room for one
   // wavefront per SIMD)
-  static constexpr bool IR = sizeof(T) == 4;
   static constexpr bool REPLAY = S::NPASS > 1 && !UQ && !IR;
-  static constexpr bool RECORD = REPLAY || IR;            // Z of every panel kept
+  static constexpr bool RECORD = REPLAY;                  // Z of every panel kept
   static constexpr bool ROT = RECORD && (R & (R - 1)) == 0;
   static constexpr int ZSTR = ROT ? R : RP;
   // column t of the Z of panel pk, row `row`: recorded panels rotate the rows
@@ -170,19 +172,20 @@ struct SchurLds {
   static HDGX_DEV constexpr int zidx(int pk, int t, int row) {
     return RECORD ? (pk * 4 + t) * ZSTR + (ROT ? ((row + t) & (R - 1)) : row) : t * RP + row;
   }
-  // scratch of the elimination phase inside sXt: panel, pivot rows and (not
-  // recorded) the panel's Z in FP64; the prologue's Cholesky factor of M
-  static constexpr int NXT = NCOLP * 16, NSCR = IR ? 0 : 4 * RP + 4 * RSTR + (RECORD ? 0 : 4 * RP);
+  // scratch inside sXt: the prologue's Cholesky factor of M; the elimination's
+  // panel, pivot rows and (not recorded) Z, in T; in FP32 the staging of
+  // D'^-1 (column j at j*NUP)
+  static constexpr int NUP = C::NU | 1;
+  static constexpr int NXT = NCOLP * 16;
+  static constexpr int NSCR = (int)(((4 * RP + 4 * RSTR + (RECORD ? 0 : 4 * RP)) * sizeof(T) + 7) / 8);
+  static constexpr int NINV = IR ? (C::NU * NUP * 4 + 7) / 8 : 0;
+  static constexpr int NSX = NXT > NSCR ? (NXT > NINV ? NXT : NINV) : (NSCR > NINV ? NSCR : NINV);
   static_assert(!UQ || NPV * NPV <= NXT, "the factor of M fits the solution tile buffer");
   static constexpr int NNMAX = (NPV > 2 * NPF) ? NPV : 2 * NPF;
   static constexpr int KSMAX = (NNMAX * NC + 3) / 4;       // k-steps of the AE product, upper bound
   // sXt (the solution columns of one RHS tile) is live only in the output
-  // phase; the factor of M (prologue) and, in FP64, the panel buffer, pivot
-  // rows and unrecorded Z share its storage.  In FP32 the panel buffer and
-  // pivot rows have their own (the refinement replays panels while sXt holds
-  // the solution)
-  double sXt[NXT > NSCR ? NXT : NSCR];
-  T sPanR[IR ? 4 * RP + 4 * RSTR : 1];
+  // phase; the scratch above shares its storage
+  double sXt[NSX];
   T sZall[RECORD ? S::NPAN * 4 * ZSTR : 1];
   double sMiRq[UQ ? NPV * C::NCU * C::ND : 1];
   double sMiC[UQ ? NPV * NPV * C::ND : 1];
@@ -248,12 +251,11 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   if (e >= A.ne) return;
 
   constexpr int RP = LD::RP, RSTR = LD::RSTR, NCOLP = LD::NCOLP;
-  constexpr bool IR = LD::IR, RECORD = LD::RECORD;
+  constexpr bool IR = LD::IR, RECORD = LD::RECORD, REPLAY = LD::REPLAY;
+  constexpr int CTPX = LD::CTPX, NUP = LD::NUP;
   constexpr int NNMAX = LD::NNMAX, KSMAX = LD::KSMAX;
   double* const sXt = L.sXt;
-  T* sPan;
-  if constexpr (IR) sPan = L.sPanR;
-  else sPan = L.sXt;
+  T* const sPan = reinterpret_cast<T*>(L.sXt);
   T* const sRows = sPan + 4 * RP;
   T* const sZall = RECORD ? L.sZall : sRows + 4 * RSTR;
   int* const sStep = L.sStep;
@@ -348,10 +350,19 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   }
   __syncthreads();
 
+  // FP32 form: A fragments of D'^-1 (row 16 rt + l%16, column 4 s + l/16),
+  // from the first pass's elimination
+  constexpr int KSU = (NU + 3) / 4;
+  T dinv[IR ? RT : 1][IR ? KSU : 1];
   for (int pass = 0; pass < S::NPASS; ++pass) {
     const int qbase = 16 * TRB * pass;       // first RHS column of this pass
-    const bool full = (pass == 0) || !RECORD;
-    V acc[RT][CTP];
+    // FP64: every pass eliminates D' (later passes replay the recorded panels
+    // when REPLAY); FP32: the first pass eliminates D', its RHS tiles and the
+    // identity, the later passes' columns start from zero and are solved by
+    // the refinement sweeps
+    const bool full = (pass == 0) || !(RECORD || IR);
+    const bool build = !IR || pass == 0;
+    V acc[RT][CTPX];
     // ---- D' tiles: D, identity on the padded diagonal
     if (full)
 #pragma unroll
@@ -366,7 +377,17 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           else if (row == col) v = 1.0;
           acc[rt][ct][i] = (T)v;
         }
-    // ---- RHS tiles: rhs0 | -F(:, trace column)
+    // ---- RHS tiles: rhs0 | -F(:, trace column); FP32: identity tiles
+    if constexpr (IR) {
+      if (pass == 0)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+          for (int ct = 0; ct < RT; ++ct)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[rt][CTP + ct][i] = (16 * rt + MT::row(i, g) == 16 * ct + c) ? (T)1 : (T)0;
+    }
+    if (build)
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
@@ -378,7 +399,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           if (row < NU && q < NRHS) v = (q == 0) ? A.Ru[e * NU + row] : -Fe[row + NU * rhs_t<C>(q)];
           acc[rt][RT + tb][i] = (T)v;
         }
-    if constexpr (UQ) {
+    if (UQ && build) {
       // D' += B M^-1 C and RHS += B M^-1 E over k-steps of B's columns, A
       // fragments (columns of B) fetched a batch of k-steps ahead
       constexpr int BB = RT >= 4 ? 2 : 4;
@@ -502,20 +523,20 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int t = 0; t < 4; ++t) {
         const int p = pvt[t];
         const int sel = 4 * (p >> 4) + MT::reg(p & 15);     // 4*rt + i
-        T tmp[CTP];
+        T tmp[CTPX];
 #pragma unroll
-        for (int ct = 0; ct < CTP; ++ct) tmp[ct] = 0;
+        for (int ct = 0; ct < CTPX; ++ct) tmp[ct] = 0;
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
           for (int i = 0; i < 4; ++i)
             if (sel == 4 * rt + i)
 #pragma unroll
-              for (int ct = 0; ct < CTP; ++ct)
+              for (int ct = 0; ct < CTPX; ++ct)
                 if (ct >= ct0) tmp[ct] = acc[rt][ct][i];
         if (g == MT::grp(p & 15))
 #pragma unroll
-          for (int ct = 0; ct < CTP; ++ct)
+          for (int ct = 0; ct < CTPX; ++ct)
             if (ct >= ct0) sRows[t * RSTR + 16 * ct + c] = tmp[ct];
       }
       __syncthreads();
@@ -524,12 +545,35 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int rt = 0; rt < RT; ++rt) {
         const T a = sZall[LD::zidx(pk, g, 16 * rt + c)];
 #pragma unroll
-        for (int ct = 0; ct < CTP; ++ct)
+        for (int ct = 0; ct < CTPX; ++ct)
           if (ct >= ct0) acc[rt][ct] = MT::op(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
       }
       __syncthreads();
     }
-    } else {
+    if constexpr (IR) {
+      // the identity tiles now hold D'^-1 with row k at pivot row p_k:
+      // staged un-permuted through LDS into the A fragments
+      T* const sInv = reinterpret_cast<T*>(sXt);
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < RT; ++ct)
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int k = sStep[16 * rt + MT::row(i, g)], col = 16 * ct + c;
+            if (k >= 0 && k < NU && col < NU) sInv[k + NUP * col] = acc[rt][CTP + ct][i];
+          }
+      __syncthreads();
+#pragma unroll
+      for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+        for (int s = 0; s < KSU; ++s) {
+          const int row = 16 * rt + c, kk = 4 * s + g;
+          dinv[rt][s] = (row < NU && kk < NU) ? sInv[row + NUP * kk] : (T)0;
+        }
+      __syncthreads();
+    }
+    } else if (REPLAY) {
       // replay the recorded panel transforms on this pass's RHS tiles
 #pragma unroll 1
       for (int pk = 0; pk < S::NPAN; ++pk) {
@@ -569,25 +613,30 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       const int q0 = qbase + 16 * tb;
       if (q0 >= NRHS) break;
       double xmx = 0.0;
+      if (build) {
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt)
+        for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int k = sStep[16 * rt + MT::row(i, g)];
-          if (k >= 0 && k < NU) {
-            const double v = (double)acc[rt][RT + tb][i];
-            sXt[k + NCOLP * c] = v;
-            xmx = fmax(xmx, fabs(v));
+          for (int i = 0; i < 4; ++i) {
+            const int k = sStep[16 * rt + MT::row(i, g)];
+            if (k >= 0 && k < NU) {
+              const double v = (double)acc[rt][RT + tb][i];
+              sXt[k + NCOLP * c] = v;
+              xmx = fmax(xmx, fabs(v));
+            }
           }
-        }
+      } else {
+        for (int r = g; r < NU; r += 4) sXt[r + NCOLP * c] = 0.0;
+      }
       __syncthreads();
       // q rows; in FP32 then FP64 refinement sweeps.  With the q rows of X
       // consistent with its u rows, the residual of the reduced system
       // D' x = rhs is that of the u equations of the element system,
       // r = [Ru | -F] - BD X, formed here in FP64 from BD and the FP64
-      // solution; the correction D'^-1 r replays the recorded FP32 panel
-      // transforms.  The sweeps stop once the predicted error |d|^2 / |d_prev|
-      // (the last contraction applied once more) is below 1e-13 |x|.
+      // solution; the correction is the FP32 product D'^-1 r.  The sweeps
+      // stop once the predicted error |d|^2 / |d_prev| (the last contraction
+      // applied once more) is below 1e-13 |x|; columns starting from zero
+      // spend their first sweep on the solve itself.
       double dprev = IR ? wave_max(xmx) : 0.0;
       bool conv = false;
 #pragma unroll 1
@@ -610,7 +659,7 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           break;
         } else {
           if (conv) break;
-          if (sweep == IR_SWEEPS) { st |= 8; break; }
+          if (sweep == IR_SWEEPS + (build ? 0 : 1)) { st |= 8; break; }
           const int q = q0 + c;
           d4 r[RT];
 #pragma unroll
@@ -634,50 +683,28 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
             ir_bd_frags<C, BR>(BDe, s0 + 2 * BR, g, c, a0);
             ir_residual_batch<C, BR>(sXt, s0 + BR, g, c, a1, r);
           }
-          // FP64 layout -> FP32 layout: row 4g + i of the FP32 tile is
-          // register g of lane 16i + c in the FP64 tile
-          V rr[RT];
+          // d = D'^-1 r in FP32: r in the FP64 accumulator layout is the B
+          // operand directly (k-step s = rows 4s + l/16 = register s%4 of
+          // row tile s/4)
+          V dl[RT];
 #pragma unroll
-          for (int rt = 0; rt < RT; ++rt)
+          for (int rt = 0; rt < RT; ++rt) dl[rt] = V{0, 0, 0, 0};
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              T v = 0;
+          for (int s = 0; s < KSU; ++s) {
+            const T b = (T)r[s / 4][s % 4];
 #pragma unroll
-              for (int j = 0; j < 4; ++j) {
-                const double w = __shfl(r[rt][j], 16 * i + c, 64);
-                if (g == j) v = (T)w;
-              }
-              rr[rt][i] = v;
-            }
-#pragma unroll 1
-          for (int pk = 0; pk < S::NPAN; ++pk) {
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-              const int p = __builtin_amdgcn_readfirstlane(sPiv[pk * 4 + t]);
-              const int sel = 4 * (p >> 4) + MT::reg(p & 15);
-              T tmp = 0;
-#pragma unroll
-              for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-                  if (sel == 4 * rt + i) tmp = rr[rt][i];
-              if (g == MT::grp(p & 15)) sRows[t * 17 + c] = tmp;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int rt = 0; rt < RT; ++rt) rr[rt] = MT::op(sZall[LD::zidx(pk, g, 16 * rt + c)], sRows[g * 17 + c], rr[rt]);
-            __syncthreads();
+            for (int rt = 0; rt < RT; ++rt) dl[rt] = MT::op(dinv[rt][s], b, dl[rt]);
           }
           double dmx = 0.0, xm = 0.0;
 #pragma unroll
           for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-              const int k = sStep[16 * rt + MT::row(i, g)];
-              if (k >= 0 && k < NU) {
-                const double d = (double)rr[rt][i];
-                const double v = sXt[k + NCOLP * c] + d;
-                sXt[k + NCOLP * c] = v;
+              const int row = 16 * rt + MT::row(i, g);
+              if (row < NU) {
+                const double d = (double)dl[rt][i];
+                const double v = sXt[row + NCOLP * c] + d;
+                sXt[row + NCOLP * c] = v;
                 dmx = fmax(dmx, fabs(d));
                 xm = fmax(xm, fabs(v));
               }
