@@ -165,16 +165,6 @@ HDGX_DEV void roe_apply(double gam, double eps, int absflag, const T* uh, const 
 }
 
 // ----------------------------------------------------------- wave helpers
-// argmax over the wavefront; ties go to the smallest lane
-HDGX_DEV void wave_argmax(double& v, int& idx) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    double ov = __shfl_xor(v, off, 64);
-    int oi = __shfl_xor(idx, off, 64);
-    if (ov > v || (ov == v && oi < idx)) { v = ov; idx = oi; }
-  }
-}
-
 HDGX_DEV int wave_or(int v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off, 64);

@@ -4,19 +4,10 @@
 
 #include "hdgx_launch.h"
 #include "quad_kernel.hip"
-#include "schur_kernel.hip"
 #include "schur_mfma.hip"
 #include "seg_kernels.hip"
 
 namespace hdgx {
-
-template <class C>
-constexpr int chunk_cols() {
-  // columns of [rhs0 | trace rhs] solved per register chunk: <= 16, balanced
-  constexpr int nrhs = 1 + C::NH;
-  constexpr int nch = (nrhs + 15) / 16;
-  return (nrhs + nch - 1) / nch;
-}
 
 template <class C>
 hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
@@ -26,11 +17,8 @@ hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, con
 
 template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
-  constexpr int CH = chunk_cols<C>();
   if (a.prec == 1)
     hipLaunchKernelGGL((schur_mfma_ir_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
-  else if (a.prec == 2)
-    hipLaunchKernelGGL((schur_kernel<C, CH, double>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
   else
     hipLaunchKernelGGL((schur_mfma_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
   return hipGetLastError();

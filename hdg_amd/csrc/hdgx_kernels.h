@@ -69,7 +69,7 @@ struct SchurArgs {
   const double *M, *C, *E, *BD, *F, *GK, *H, *Rq, *Ru, *Rh;
   double *dudg, *dudg_duh, *ae, *fe;
   int32_t* status;
-  int32_t prec;                    // 0 fp64, 1 fp32 + fp64 refinement
+  int32_t prec;                    // 0 fp64, 1 fp32 mfma + fp64 refinement
 };
 
 // One request's element range inside a batched launch: its arrays and the

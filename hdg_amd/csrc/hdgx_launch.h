@@ -19,7 +19,8 @@
 namespace hdgx {
 
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
-// prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 + FP64 refinement; 2: FP64 row-per-lane LU.
+// prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 MFMA Gauss-Jordan (with D'^-1) + FP64
+// refinement.
 // perm (device, may be null) marks GK as face sparse.
 template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm);
 // batched launches over a segment table (FP64 MFMA elimination)
