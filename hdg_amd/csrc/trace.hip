@@ -170,9 +170,13 @@ __global__ __launch_bounds__(64) void trace_assemble_kernel(int ncu, int npf, in
 // ---- y = x + sum_k K_k x(f2f(k)) (gmres.m:38-42, arraymul.c:53-70); NCF/NBF
 // fix the block shape at compile time (0: the runtime values)
 template <int NCF, int NBF>
+// with nrm2: the operand is x / sqrt(*nrm2) (a Krylov vector not yet
+// normalised), y = A x / sqrt(*nrm2), and the normalised vector is written to
+// vout for this face's own rows
 __global__ __launch_bounds__(256) void trace_matvec_kernel(int ncf_, int nbf_, int64_t nf, const double* __restrict__ K,
                                                            const int32_t* __restrict__ f2f,
                                                            const double* __restrict__ x, double* __restrict__ y,
+                                                           const double* __restrict__ nrm2, double* __restrict__ vout,
                                                            const int* __restrict__ stop) {
   if (stop && *stop) return;
   const int ncf = NCF ? NCF : ncf_, nbf = NBF ? NBF : nbf_;
@@ -193,7 +197,14 @@ __global__ __launch_bounds__(256) void trace_matvec_kernel(int ncf_, int nbf_, i
 #pragma unroll
     for (int c = 0; c < ncf; ++c) acc += Kk[r + ncf * c] * xj[c];
   }
-  y[r + (int64_t)ncf * i] = x[r + (int64_t)ncf * i] + acc;
+  const double xi = x[r + (int64_t)ncf * i];
+  if (nrm2) {
+    const double sc = 1.0 / sqrt(*nrm2);
+    y[r + (int64_t)ncf * i] = (xi + acc) * sc;
+    vout[r + (int64_t)ncf * i] = xi * sc;
+  } else {
+    y[r + (int64_t)ncf * i] = xi + acc;
+  }
 }
 
 // ---- block ILU(0), one row of the current colour per workgroup
@@ -289,30 +300,48 @@ __global__ __launch_bounds__(256) void bilu0_lower_kernel(int ncf_, int nbf_, co
                                                           const int32_t* __restrict__ f2f,
                                                           const int32_t* __restrict__ rank,
                                                           const double* __restrict__ LU, const double* __restrict__ x,
-                                                          double* __restrict__ y, const int* __restrict__ stop) {
+                                                          double* __restrict__ y, int n0, double* __restrict__ vd,
+                                                          const int* __restrict__ stop) {
   if (stop && *stop) return;
+  __shared__ double tmp[256];
   const int ncf = NCF ? NCF : ncf_, nbf = NBF ? NBF : nbf_;
   const int fpb = blockDim.x / ncf;
   const int r = threadIdx.x % ncf, fl = threadIdx.x / ncf;
-  if (fl >= fpb) return;
   const int64_t ri = (int64_t)blockIdx.x * fpb + fl;
-  if (ri >= nrows) return;
-  const int i = rows[ri];
-  const int n2 = ncf * ncf, rk = rank[i];
+  const bool active = fl < fpb && ri < nrows;
+  const int i = active ? rows[ri] : 0;
+  const int n2 = ncf * ncf;
   const double* LUi = LU + (size_t)n2 * nbf * i;
-  double v = x[r + (int64_t)ncf * i];
+  double v = 0.0;
+  if (active) {
+    const int rk = rank[i];
+    v = x[r + (int64_t)ncf * i];
 #pragma unroll
-  for (int k = 1; k < nbf; ++k) {
-    const int j = f2f[k + (int64_t)nbf * i];
-    if (j < 0 || rank[j] > rk) continue;
-    const double* L = LUi + (size_t)n2 * k;
-    const double* yj = y + (size_t)ncf * j;
+    for (int k = 1; k < nbf; ++k) {
+      const int j = f2f[k + (int64_t)nbf * i];
+      if (j < 0 || rank[j] > rk) continue;
+      const double* L = LUi + (size_t)n2 * k;
+      // the first colour has no earlier neighbours, so its y is x
+      const double* yj = (rank[j] < n0 ? x : y) + (size_t)ncf * j;
+      double s = 0.0;
+#pragma unroll
+      for (int c = 0; c < ncf; ++c) s += L[r + ncf * c] * yj[c];
+      v -= s;
+    }
+  }
+  if (!vd) {
+    if (active) y[r + (int64_t)ncf * i] = v;
+    return;
+  }
+  // last colour: no later neighbours, so the upper solve is v = U_ii^-1 y
+  tmp[threadIdx.x] = v;
+  __syncthreads();
+  if (active) {
     double s = 0.0;
 #pragma unroll
-    for (int c = 0; c < ncf; ++c) s += L[r + ncf * c] * yj[c];
-    v -= s;
+    for (int c = 0; c < ncf; ++c) s += LUi[r + ncf * c] * tmp[fl * ncf + c];
+    vd[r + (int64_t)ncf * i] = s;
   }
-  y[r + (int64_t)ncf * i] = v;
 }
 
 // backward sweep over one colour: v_i = U_ii^-1 (y_i - sum_{later k} U_ik v_k)
@@ -642,7 +671,7 @@ int ensure_tws(hdgx_ctx* ctx, size_t bytes) {
 }
 
 int launch_matvec(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double* x, double* y,
-                  const int* stop = nullptr) {
+                  const int* stop = nullptr, const double* nrm2 = nullptr, double* vout = nullptr) {
   const int ncf = T->ncu * T->npf, nbf = 2 * T->nfe - 1;
   const int fpb = 256 / ncf;
   const unsigned grid = (unsigned)((T->nf + fpb - 1) / fpb);
@@ -651,14 +680,14 @@ int launch_matvec(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const dou
 #define HDGX_MV(NC, NB)                                                                                           \
     if (!done && ncf == NC && nbf == NB) {                                                                         \
       hipLaunchKernelGGL((trace_matvec_kernel<NC, NB>), dim3(grid), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf,   \
-                         T->nf, K, T->f2f, x, y, stop);                                                            \
+                         T->nf, K, T->f2f, x, y, nrm2, vout, stop);                                                \
       done = true;                                                                                                 \
     }
     HDGX_TRACE_SHAPES(HDGX_MV)
 #undef HDGX_MV
     if (!done)
       hipLaunchKernelGGL((trace_matvec_kernel<0, 0>), dim3(grid), dim3(fpb * ncf), 0, ctx->stream, ncf, nbf, T->nf, K,
-                         T->f2f, x, y, stop);
+                         T->f2f, x, y, nrm2, vout, stop);
   }
   HIP_TRY(ctx, hipGetLastError());
   return HDGX_OK;
@@ -668,32 +697,41 @@ int launch_bilu0_apply(hdgx_ctx* ctx, const hdgx_trace* T, const double* LU, con
                        const int* stop = nullptr) {
   const int ncf = T->ncu * T->npf, nbf = 2 * T->nfe - 1;
   const int fpb = 256 / ncf;
-  auto sweep = [&](bool lower, int c) {
+  const int C = T->ncolors;
+  if (C < 1) return HDGX_OK;
+  const int n0 = T->color_ptr[1] - T->color_ptr[0];
+  // lower sweep of colour c (the last one also applies U_ii^-1 into y) or
+  // upper sweep of colour c reading the lower solution from ysrc
+  auto sweep = [&](bool lower, int c, bool last, const double* ysrc) {
     const int n = T->color_ptr[c + 1] - T->color_ptr[c];
     if (n == 0) return;
     const int32_t* rows = T->color_rows + T->color_ptr[c];
     const dim3 grid((n + fpb - 1) / fpb), block(fpb * ncf);
+    double* vd = last ? y : nullptr;
 #define HDGX_SW(NC, NB)                                                                                           \
     if (ncf == NC && nbf == NB) {                                                                                  \
       if (lower)                                                                                                   \
         hipLaunchKernelGGL((bilu0_lower_kernel<NC, NB>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,  \
-                           T->face_rank, LU, x, tmp, stop);                                                        \
+                           T->face_rank, LU, x, tmp, n0, vd, stop);                                                \
       else                                                                                                         \
         hipLaunchKernelGGL((bilu0_upper_kernel<NC, NB>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,  \
-                           T->face_rank, LU, tmp, y, stop);                                                        \
+                           T->face_rank, LU, ysrc, y, stop);                                                       \
       return;                                                                                                      \
     }
     HDGX_TRACE_SHAPES(HDGX_SW)
 #undef HDGX_SW
     if (lower)
       hipLaunchKernelGGL((bilu0_lower_kernel<0, 0>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,
-                         T->face_rank, LU, x, tmp, stop);
+                         T->face_rank, LU, x, tmp, n0, vd, stop);
     else
       hipLaunchKernelGGL((bilu0_upper_kernel<0, 0>), grid, block, 0, ctx->stream, ncf, nbf, rows, n, T->f2f,
-                         T->face_rank, LU, tmp, y, stop);
+                         T->face_rank, LU, ysrc, y, stop);
   };
-  for (int c = 0; c < T->ncolors; ++c) sweep(true, c);
-  for (int c = T->ncolors - 1; c >= 0; --c) sweep(false, c);
+  // the first colour's lower solve is the identity (no earlier neighbours):
+  // later colours read its y from x; the last colour's upper solve is its
+  // diagonal block alone and runs inside its lower sweep
+  for (int c = C > 1 ? 1 : 0; c < C; ++c) sweep(true, c, c == C - 1, nullptr);
+  for (int c = C - 2; c >= 0; --c) sweep(false, c, false, c == 0 ? x : tmp);
   HIP_TRY(ctx, hipGetLastError());
   return HDGX_OK;
 }
@@ -793,15 +831,16 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
   const int64_t N = (int64_t)T->ncu * T->npf * T->nf;
   if (maxit <= 0) maxit = (int)std::min<int64_t>(N, 1 << 30);
   const int m1 = restart + 1;
-  // workspace: V (N x m1) | t | u | b~ | partial | h | y | Arnoldi state
+  // workspace: V (N x m1) | t | u | b~ | w0 | w1 | partial | h | y | Arnoldi state
   const size_t dwords = (sizeof(GmresDev) + 7) / 8;
-  const size_t words = (size_t)N * (m1 + 3) + (size_t)RED_BLOCKS * MAXM + 2 * MAXM + dwords;
+  const size_t words = (size_t)N * (m1 + 5) + (size_t)RED_BLOCKS * MAXM + 2 * MAXM + dwords;
   if ((rc = ensure_tws(ctx, words * sizeof(double)))) return rc;
   double* V = static_cast<double*>(ctx->tws);
   double* t = V + (size_t)N * m1;
   double* u = t + N;
   double* bt = u + N;
-  double* partial = bt + N;
+  double* wk[2] = {bt + N, bt + 2 * N};     // Krylov vectors before normalisation
+  double* partial = bt + 3 * N;
   double* hdev = partial + (size_t)RED_BLOCKS * MAXM;
   double* ydev = hdev + MAXM;
   GmresDev* dst = reinterpret_cast<GmresDev*>(ydev + MAXM);
@@ -810,10 +849,12 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
   hipStream_t s = ctx->stream;
   constexpr int SYNC_EVERY = 4;     // Arnoldi steps launched between host syncs
 
-  // operator: out = A v (plus the ILU(0) solve when LU is given)
-  auto op = [&](const double* v, double* out, const int* stp) -> int {
-    if (!LU) return launch_matvec(ctx, T, K, v, out, stp);
-    int r = launch_matvec(ctx, T, K, v, u, stp);
+  // operator: out = A v (plus the ILU(0) solve when LU is given); with nrm2
+  // the operand is v / sqrt(*nrm2), written normalised to vout
+  auto op = [&](const double* v, double* out, const int* stp, const double* nrm2 = nullptr,
+                double* vout = nullptr) -> int {
+    if (!LU) return launch_matvec(ctx, T, K, v, out, stp, nrm2, vout);
+    int r = launch_matvec(ctx, T, K, v, u, stp, nrm2, vout);
     if (r) return r;
     return launch_bilu0_apply(ctx, T, LU, u, out, t, stp);   // out is never t
   };
@@ -856,9 +897,12 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
     if (fl < 0) {
       hipLaunchKernelGGL(cycle_init_kernel, dim3(1), dim3(64), 0, s, dst, beta, it);
       for (int j = 0; j < restart; ++j) {
-        double* vj = V + (size_t)N * j;
-        double* w = V + (size_t)N * (j + 1);
-        if ((rc = op(vj, w, stop))) return rc;
+        // step j's operand: V(:,0), or the previous step's w, normalised by
+        // the matvec into V(:,j) with |w|^2 from hcol[j]
+        double* w = wk[j & 1];
+        if (j == 0) rc = op(V, w, stop);
+        else rc = op(wk[(j - 1) & 1], w, stop, hdev + j, V + (size_t)N * j);
+        if (rc) return rc;
         // h(0:j) = V(:,0:j)' w ; w -= V h ; |w|^2 (gmres.m:80-83)
         hipLaunchKernelGGL(dots_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, j + 1, (const double*)V, N,
                            (const double*)w, partial, stop);
@@ -867,8 +911,6 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
                            (const double*)hdev, w, partial + (size_t)RED_BLOCKS * (MAXM - 1), stop);
         hipLaunchKernelGGL(finish_tail_kernel, dim3(1), dim3(256), 0, s, (int)rg,
                            (const double*)(partial + (size_t)RED_BLOCKS * (MAXM - 1)), hdev, dst, j, tolb, maxit);
-        hipLaunchKernelGGL(scale_kernel, dim3(rg), dim3(RED_THREADS), 0, s, N, (const double*)w,
-                           (const double*)(hdev + j + 1), w, stop);
         HIP_TRY(ctx, hipGetLastError());
         if ((j + 1) % SYNC_EVERY == 0 || j == restart - 1) {
           HIP_TRY(ctx, hipMemcpyAsync(hst, dst, offsetof(GmresDev, H), hipMemcpyDeviceToHost, s));
