@@ -59,6 +59,52 @@ constexpr int IR_SWEEPS = 4;
 // no column: entry of the AE k-step table past an AE row tile's node set
 constexpr int AE_NONE = 255;
 
+// value of lane `src` (wavefront-uniform) in every lane, through v_readlane
+HDGX_DEV double lane_bcast(double v, int src) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+  return __hiloint2double(hi, lo);
+}
+HDGX_DEV float lane_bcast(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+
+// maximum of a pivot-search key over the wavefront, wavefront-uniform: DPP
+// rotations within each row of 16 lanes, then the four row maxima read out
+template <int SH>
+HDGX_DEV unsigned row_ror(unsigned v) {
+  return (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x120 + SH, 0xf, 0xf, false);
+}
+template <int SH>
+HDGX_DEV unsigned long long key_max_ror(unsigned long long k) {
+  const unsigned long long o = ((unsigned long long)row_ror<SH>((unsigned)(k >> 32)) << 32) | row_ror<SH>((unsigned)k);
+  return o > k ? o : k;
+}
+template <int SH>
+HDGX_DEV unsigned key_max_ror(unsigned k) {
+  const unsigned o = row_ror<SH>(k);
+  return o > k ? o : k;
+}
+template <typename K>
+HDGX_DEV K wave_key_max(K k) {
+  k = key_max_ror<8>(k);
+  k = key_max_ror<4>(k);
+  k = key_max_ror<2>(k);
+  k = key_max_ror<1>(k);
+  K m = 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    K v;
+    if constexpr (sizeof(K) == 8)
+      v = ((K)(unsigned)__builtin_amdgcn_readlane((int)(k >> 32), 16 * r) << 32) |
+          (unsigned)__builtin_amdgcn_readlane((int)k, 16 * r);
+    else
+      v = (unsigned)__builtin_amdgcn_readlane((int)k, 16 * r);
+    m = v > m ? v : m;
+  }
+  return m;
+}
+
 // maximum over the wavefront
 HDGX_DEV double wave_max(double v) {
 #pragma unroll
@@ -219,13 +265,25 @@ template <class C, int BR>
 HDGX_DEV void ir_residual_batch(const double* sXt, int s0, int g, int c, const double (&a)[BR][MfmaShape<C>::RT],
                                 d4 (&r)[MfmaShape<C>::RT]) {
   constexpr int NCOL = C::NCOL, NCOLP = C::NCOL | 1, RT = MfmaShape<C>::RT;
+  constexpr int KS = (NCOL + 3) / 4;
 #pragma unroll
   for (int sb = 0; sb < BR; ++sb) {
+    if (s0 + sb >= KS) break;
     const int k = 4 * (s0 + sb) + g;
     const double b = (k < NCOL) ? -sXt[k + NCOLP * c] : 0.0;
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt) r[rt] = mfma64(a[sb][rt], b, r[rt]);
   }
+}
+
+// AE = H + GK X: the column of GK and X that k-step s of AE row tile rta
+// reads in lane group g (-1 past the tile's node set)
+template <class C>
+HDGX_DEV int ae_col(const unsigned char* sAeK, int rta, int s, int g) {
+  constexpr int KSMAX = SchurLds<C>::KSMAX;
+  if (s >= KSMAX) return -1;
+  const int k = sAeK[rta * 4 * KSMAX + 4 * s + g];
+  return k == AE_NONE ? -1 : k;
 }
 
 // Element e of the batch A (one 64-lane wavefront), LDS in L.
@@ -482,20 +540,16 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
         // max-reduction finds the largest magnitude and, on ties, the lowest lane
         K key = 0;
         if (lane < R && !done) key = (MT::key(fabs(pr[t])) & ~(K)63) | (K)(63 - lane);
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-          const K o = __shfl_xor(key, off, 64);
-          key = o > key ? o : key;
-        }
+        key = wave_key_max(key);
         if ((key >> 6) == 0) st |= 2;
         const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63));
         pvt[t] = p;
-        const T pv = __shfl(pr[t], p, 64);
+        const T pv = lane_bcast(pr[t], p);
         const T ip = (T)1 / pv;
         z[t] = (lane == p) ? (T)1 : (T)0;
         T ppr[4], pz[4];
 #pragma unroll
-        for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = __shfl(pr[t2], p, 64); pz[t2] = __shfl(z[t2], p, 64); }
+        for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = lane_bcast(pr[t2], p); pz[t2] = lane_bcast(z[t2], p); }
         if (lane == p) {
 #pragma unroll
           for (int t2 = 0; t2 < 4; ++t2) { pr[t2] *= ip; z[t2] *= ip; }
@@ -750,17 +804,10 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           for (int sb = 0; sb < BATCH; ++sb)
 #pragma unroll
             for (int rta = 0; rta < RTH; ++rta) {
-              a[sb][rta] = 0.0;
-              b[sb][rta] = 0.0;
-              if (s0 + sb < KSMAX) {
-                int K = sAeK[rta * 4 * KSMAX + 4 * (s0 + sb) + g];
-                if (K == AE_NONE) K = -1;
-                const int row_a = 16 * rta + c;
-                if (K >= 0) {
-                  if (row_a < NH) a[sb][rta] = GKe[row_a + NH * K];
-                  b[sb][rta] = sXt[K + NCOLP * c];
-                }
-              }
+              const int k = ae_col<C>(sAeK, rta, s0 + sb, g);
+              const int row_a = 16 * rta + c;
+              a[sb][rta] = (k >= 0 && row_a < NH) ? GKe[row_a + NH * k] : 0.0;
+              b[sb][rta] = k >= 0 ? sXt[k + NCOLP * c] : 0.0;
             }
 #pragma unroll
           for (int sb = 0; sb < BATCH; ++sb)
