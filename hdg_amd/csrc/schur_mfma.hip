@@ -794,11 +794,14 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
             ae[rta][i] = (row < NH && q >= 1 && q < NRHS) ? He[row + NH * rhs_t<C>(q)] : 0.0;
           }
         constexpr int BATCH = RTH >= 4 ? 2 : 4;
-        int klen[RTH];
+        int klen[RTH], kend = 0;     // k-steps that meet a tile's node set
 #pragma unroll
-        for (int rta = 0; rta < RTH; ++rta) klen[rta] = __builtin_amdgcn_readfirstlane(sAeNN[rta] * NC);
+        for (int rta = 0; rta < RTH; ++rta) {
+          klen[rta] = __builtin_amdgcn_readfirstlane(sAeNN[rta] * NC);
+          kend = max(kend, (klen[rta] + 3) / 4);
+        }
 #pragma unroll 1
-        for (int s0 = 0; s0 < KSMAX; s0 += BATCH) {
+        for (int s0 = 0; s0 < kend; s0 += BATCH) {
           double a[BATCH][RTH], b[BATCH][RTH];
 #pragma unroll
           for (int sb = 0; sb < BATCH; ++sb)
