@@ -202,17 +202,32 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   double* const sC = L.sC;
   double* const sE = L.sE;
 
-  // ---- 0. stage the element's inputs and the small master tables
-  for (int i = tid; i < NPV * ncd; i += NT) sX[i] = A.dgnodes[e * NPV * ncd + i];
-  for (int i = tid; i < NPV * NC; i += NT) sU[i] = A.udg[e * NPV * NC + i];
-  for (int i = tid; i < NH; i += NT) sUH[i] = A.uh[e * NH + i];
-  if (UQ || tdep)
-    for (int i = tid; i < NPV * NC; i += NT) sSH[i] = A.sh ? A.sh[e * NPV * NC + i] : 0.0;
-  if (tid < NFE) sBf[tid] = A.bf[e * NFE + tid];
-  for (int i = tid; i < NPV * NFE; i += NT) sLoc[i] = T.loc[i];
-  for (int i = tid; i < NGV * NPV; i += NT) sShV[i] = T.shapvt[i];
-  for (int i = tid; i < NGF * NPF; i += NT) sShF[i] = T.shapft[i];
-  for (int i = tid; i < NPF * NPF * NGF; i += NT) sSff[i] = T.sff[i];
+  // ---- 0. stage the element's inputs and the small master tables: every
+  // array fits one pass of the workgroup, so all loads are issued before
+  // the first LDS store (one memory latency instead of one per array)
+  {
+    static_assert(NPV * (ND + 2) <= NT && NPV * NC <= NT && NH <= NT && NPV * NFE <= NT && NGV * NPV <= NT &&
+                  NGF * NPF <= NT && NPF * NPF * NGF <= NT, "staged arrays fit one workgroup pass");
+    const bool sh_on = UQ || tdep;
+    const double x = tid < NPV * ncd ? A.dgnodes[e * NPV * ncd + tid] : 0.0;
+    const double u = tid < NPV * NC ? A.udg[e * NPV * NC + tid] : 0.0;
+    const double uh = tid < NH ? A.uh[e * NH + tid] : 0.0;
+    const double sh = (sh_on && A.sh && tid < NPV * NC) ? A.sh[e * NPV * NC + tid] : 0.0;
+    const int32_t bf = tid < NFE ? A.bf[e * NFE + tid] : 0;
+    const int32_t loc = tid < NPV * NFE ? T.loc[tid] : 0;
+    const double shv = tid < NGV * NPV ? T.shapvt[tid] : 0.0;
+    const double shf = tid < NGF * NPF ? T.shapft[tid] : 0.0;
+    const double sff = tid < NPF * NPF * NGF ? T.sff[tid] : 0.0;
+    if (tid < NPV * ncd) sX[tid] = x;
+    if (tid < NPV * NC) sU[tid] = u;
+    if (tid < NH) sUH[tid] = uh;
+    if (sh_on && tid < NPV * NC) sSH[tid] = sh;
+    if (tid < NFE) sBf[tid] = bf;
+    if (tid < NPV * NFE) sLoc[tid] = loc;
+    if (tid < NGV * NPV) sShV[tid] = shv;
+    if (tid < NGF * NPF) sShF[tid] = shf;
+    if (tid < NPF * NPF * NGF) sSff[tid] = sff;
+  }
   __syncthreads();
 
   // ---- 1. pointwise inputs
@@ -453,11 +468,13 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   for (int idx = tid; idx < NU; idx += NT) {
     const int a = idx % NPV, c = idx / NPV;
     double v = 0.0;
+#pragma unroll
     for (int r = 0; r < NR; ++r) v += T.shapvg[a + NPV * r] * sWrk[r + NR * c];
 #pragma unroll
     for (int j = 0; j < NFE; ++j) {
       const int la = sLoc[a + NPV * j];
       if (la >= 0)
+#pragma unroll
         for (int g = 0; g < NGF; ++g) v -= T.shapfg[la + NPF * g] * sFH[j * NGF + g + NFG * c];
     }
     Ru[idx] = v;
@@ -467,6 +484,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   for (int idx = tid; idx < NH; idx += NT) {
     const int c = idx % NCU, ab = idx / NCU, j = ab / NPF, a = ab - j * NPF;
     double v = 0.0;
+#pragma unroll
     for (int g = 0; g < NGF; ++g) v += T.shapfg[a + NPF * g] * sGH[j * NGF + g + NFG * c];
     Rh[idx] = -v;
   }
@@ -521,6 +539,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     // ---- 3g. M, C (volint.m:43-55), E (faceint.m:34-47)
     for (int idx = tid; idx < NPV * NPV; idx += NT) {
       double v = 0.0;
+#pragma unroll
       for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
       v *= ap.fc_q;
       sM[idx] = v;
@@ -531,6 +550,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       double v = 0.0;
 #pragma unroll
       for (int j = 0; j < ND; ++j)
+#pragma unroll
         for (int g = 0; g < NGV; ++g) v += T.svv[jjii + NPV * NPV * (g + NGV * (j + 1))] * sXx[g * ND * ND + i * ND + j];
       sC[idx] = v;
       A.C[e * NPV * NPV * ND + idx] = v;
