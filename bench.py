@@ -59,8 +59,9 @@ def time_steps(torch, stream, fn, steps, warmup):
 def kernel_times(torch, ctx, P, dev, out, reps):
     """Per-kernel durations with events on the launch stream, each over `reps`
     back-to-back launches so that the host's launch work overlaps the device's:
-    the quadrature kernel alone (hdgx_elemental_matrices) and the elimination
-    kernel as hdgx_elemental runs it (the step minus the quadrature)."""
+    the quadrature kernel alone (hdgx_elemental_matrices into the same arrays
+    every time) and the elimination kernel as hdgx_elemental runs it (the step
+    minus the quadrature)."""
     s = ctx.stream
     mats = ctx.elemental_matrices(P.master, P.app, *dev)
     ctx.elemental(P.master, P.app, *dev, out=out)
@@ -68,7 +69,7 @@ def kernel_times(torch, ctx, P, dev, out, reps):
     e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
     e[0].record(s)
     for _ in range(reps):
-        mats = ctx.elemental_matrices(P.master, P.app, *dev)
+        ctx.elemental_matrices(P.master, P.app, *dev, out=mats)
     e[1].record(s)
     torch.cuda.synchronize()
     del mats

@@ -238,7 +238,10 @@ class Context:
                               ctypes.cast(ctypes.pointer(ca), ctypes.c_void_p).value)
         self._check(lib().hdgx_elemental_batch(self.h, ctypes.byref(mc.c), len(segments), arr))
 
-    def elemental_matrices(self, master, app, dgnodes, bf, udg, uh, sh=None):
+    def elemental_matrices(self, master, app, dgnodes, bf, udg, uh, sh=None, out=None):
+        """The element matrices as volint + faceint hand them to schur_primal
+        (BD, F, GK, H, Ru, Rh and, with q, M, C, E, Rq); `out` (a dict of a
+        previous call's arrays) is written in place."""
         ca = app.to_c() if isinstance(app, App) else app
         mc = self.master_c(master)
         dg, b, u, h = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh)
@@ -246,12 +249,14 @@ class Context:
         ne = dg.shape[2]
         npv, ncu, nc, nd = master.npv, ca.ncu, ca.nc, master.nd
         ndf = master.npf * master.nfe
-        out = {"BD": Dev.empty((npv, ncu, npv, nc, ne)), "F": Dev.empty((npv, ncu, ncu, ndf, ne)),
-               "GK": Dev.empty((ncu, ndf, npv, nc, ne)), "H": Dev.empty((ncu, ndf, ncu, ndf, ne)),
-               "Ru": Dev.empty((npv, ncu, ne)), "Rh": Dev.empty((ncu, ndf, ne))}
+        shapes = {"BD": (npv, ncu, npv, nc, ne), "F": (npv, ncu, ncu, ndf, ne), "GK": (ncu, ndf, npv, nc, ne),
+                  "H": (ncu, ndf, ncu, ndf, ne), "Ru": (npv, ncu, ne), "Rh": (ncu, ndf, ne)}
         if ca.localsolve:
-            out.update(M=Dev.empty((npv, npv, ne)), C=Dev.empty((npv, npv, nd, ne)),
-                       E=Dev.empty((npv, ndf, nd, ne)), Rq=Dev.empty((npv, ncu, nd, ne)))
+            shapes.update(M=(npv, npv, ne), C=(npv, npv, nd, ne), E=(npv, ndf, nd, ne), Rq=(npv, ncu, nd, ne))
+        if out is None:
+            out = {k: Dev.empty(v) for k, v in shapes.items()}
+        elif set(out) != set(shapes) or any(tuple(out[k].shape) != v for k, v in shapes.items()):
+            raise HdgxError("elemental_matrices: out does not match these sizes")
         ptrs = [_p(out.get(k)) for k in ("BD", "F", "GK", "H", "Ru", "Rh", "M", "C", "E", "Rq")]
         self._check(lib().hdgx_elemental_matrices(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr,
                                                   u.ptr, h.ptr, _p(s), *ptrs))
