@@ -107,13 +107,27 @@ def test_gmres_solves_the_trace_system(ctx, oracle, case, precond):
         assert abs(it - it_ref) <= 2, (it, it_ref)
 
 
+@pytest.mark.parametrize("ordering", ["colors", "natural"])
 @pytest.mark.parametrize("case", ["euler2d_p2", "ns2d_p3", "euler3d_p1"])
-def test_bilu0_factor_and_apply(ctx, oracle, case):
+def test_bilu0_factor_and_apply(ctx, oracle, case, ordering):
+    """Block ILU(0) factor and apply against computeBILU0/applyBILU0 in the
+    same row order: the greedy face colouring (3 colours on triangles, more on
+    tetrahedra) or one colour per face (the natural order, every sweep one
+    face: the apply's first-colour, middle-colour and last-colour paths)."""
     from hdg_amd.runtime import Dev
-    from hdg_amd.trace import TraceSystem
+    from hdg_amd.trace import TraceSystem, face_connectivity, mkf2f
     P, udg, UH, AE, FE, *_ = _system(oracle, case)
-    print(f"[gpu] bilu0 {case}", flush=True)
-    ts = TraceSystem(ctx, P.mesh, P.app.ncu)
+    print(f"[gpu] bilu0 {case} {ordering}", flush=True)
+    conn = None
+    if ordering == "natural":
+        f2f = mkf2f(P.mesh)
+        nf = f2f.shape[1]
+        ids = np.arange(nf, dtype=np.int32)
+        conn = dict(fconn=face_connectivity(P.mesh), f2f=f2f,
+                    color=(np.arange(nf + 1, dtype=np.int32), ids, ids, ids))
+    ts = TraceSystem(ctx, P.mesh, P.app.ncu, conn=conn)
+    if ordering == "natural":
+        assert ts.ncolors == ts.nf
     K, F, st = ts.assemble(Dev.from_numpy(AE), Dev.from_numpy(FE))
     LU, lst = ts.bilu0_factor(K)
     x = np.random.default_rng(8).standard_normal((ts.ncf, ts.nf))
