@@ -696,16 +696,27 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
 #pragma unroll 1
       for (int sweep = 0;; ++sweep) {
         if constexpr (UQ) {
-          // dq = MiRq + MiC du ; dlq = MiC dlu - MiE (x) I   (schur_uq.c:146-186)
+          // dq = MiRq + MiC du ; dlq = MiC dlu - MiE (x) I   (schur_uq.c:146-186):
+          // the q rows n + NPV (m + NCU dd) of component m and direction dd
+          // per lane group, with that component's du of the lane's column
+          // loaded once
           const int q = q0 + c;
-          for (int rr = g; rr < NQ; rr += 4) {
-            const int n = rr % NPV, m = (rr / NPV) % NCU, dd = rr / (NPV * NCU);
-            double v = 0.0;
+          for (int bq = g; bq < NCU * ND; bq += 4) {
+            const int m = bq % NCU, dd = bq / NCU;
+            double xu[NPV];
 #pragma unroll
-            for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * sXt[a + NPV * m + NCOLP * c];
-            if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
-            else if (q < NRHS && rhs_k<C>(q) == m) v -= sMiE[n + NPV * rhs_jf<C>(q) + NPV * NDF * dd];
-            sXt[NU + rr + NCOLP * c] = v;
+            for (int a = 0; a < NPV; ++a) xu[a] = sXt[a + NPV * m + NCOLP * c];
+            const bool diag = q >= 1 && q < NRHS && rhs_k<C>(q) == m;
+            const int jf = diag ? rhs_jf<C>(q) : 0;
+#pragma unroll
+            for (int n = 0; n < NPV; ++n) {
+              double v = 0.0;
+#pragma unroll
+              for (int a = 0; a < NPV; ++a) v += sMiC[n + NPV * a + NPV * NPV * dd] * xu[a];
+              if (q == 0) v += sMiRq[n + NPV * m + NPV * NCU * dd];
+              else if (diag) v -= sMiE[n + NPV * jf + NPV * NDF * dd];
+              sXt[NU + n + NPV * (m + NCU * dd) + NCOLP * c] = v;
+            }
           }
           __syncthreads();
         }
