@@ -66,6 +66,12 @@ HDGX_DEV double dabs(double a) { return fabs(a); }
 template <int N> HDGX_DEV double val(const Dual<N>& a) { return a.v; }
 HDGX_DEV double val(double a) { return a; }
 
+// type of an expression in operands of types A and B (double or one Dual type):
+// double only when both are
+template <class A, class B> struct Prom { typedef A type; };
+template <class B> struct Prom<double, B> { typedef B type; };
+template <class A, class B> using prom_t = typename Prom<A, B>::type;
+
 // ------------------------------------------------------------------ physics
 enum { PHYS_EULER = 0, PHYS_NS = 1 };
 
@@ -119,8 +125,8 @@ HDGX_DEV void ns2d_flux(double gam, double Re, double Pr, const TU* u, const TQ*
 // the matrix: A_n = rl3*I + a*cc1^T + e*cc2^T with a = (1, vel, h),
 // e = (0, n, un) (ns/getan.m, fbou_euler3d.c); absflag 1 gives |A_n| with the
 // entropy fix of parameter eps.
-template <int ND, class T>
-HDGX_DEV void roe_apply(double gam, double eps, int absflag, const T* uh, const double* n, const T* x, T* y) {
+template <int ND, class T, class TX, class TY>
+HDGX_DEV void roe_apply(double gam, double eps, int absflag, const T* uh, const double* n, const TX* x, TY* y) {
   const double gam1 = gam - 1.0;
   constexpr int NC = ND + 2;
   T r1 = 1.0 / uh[0];
@@ -150,7 +156,7 @@ HDGX_DEV void roe_apply(double gam, double eps, int absflag, const T* uh, const 
   T A1 = gam1 * sm * ic2, A2 = s2 * ic, B1 = gam1 * s2 * ic;
   // d1 = cc1 . x, d2 = cc2 . x with b = (af, -vel, 1), d = (-un, n, 0):
   // cc1 = A1*b + A2*d, cc2 = B1*b + sm*d
-  T d1 = (A1 * af - A2 * un) * x[0], d2 = (B1 * af - sm * un) * x[0];
+  TY d1 = (A1 * af - A2 * un) * x[0], d2 = (B1 * af - sm * un) * x[0];
 #pragma unroll
   for (int k = 1; k <= ND; ++k) {
     d1 = d1 + (A2 * n[k - 1] - A1 * vel[k - 1]) * x[k];

@@ -57,16 +57,20 @@ HDGX_DEV void vol_flux(const DevApp& ap, double av, const T* u, T f[C::NCU][C::N
 }
 
 // fhat = f(uh[, q]).n + tau (u - uh) (euler/fhat.m caseFlux 1 / caseStab 4,
-// ns/fhat.m:49-67; AV adds av q.n)
-template <class C, class T>
-HDGX_DEV void face_fhat(const DevApp& ap, double av, const double* n, const T* u, const T* uh, T fh[C::NCU]) {
+// ns/fhat.m:49-67; AV adds av q.n).  u and uh may differ in type (one of them
+// carries the derivative direction, the other is constant).
+template <class C, class TU, class TH>
+HDGX_DEV void face_fhat(const DevApp& ap, double av, const double* n, const TU* u, const TH* uh,
+                        prom_t<TU, TH> fh[C::NCU]) {
   constexpr int ND = C::ND, NCU = C::NCU;
-  T fl[NCU][ND];
+  // the flux of uh (and, with viscosity, of q from u)
+  typedef typename std::conditional<C::PHYS == PHYS_EULER, TH, prom_t<TU, TH>>::type TF;
+  TF fl[NCU][ND];
   if constexpr (C::PHYS == PHYS_EULER) euler_flux<ND>(ap.gam, uh, fl);
   else ns2d_flux(ap.gam, ap.Re, ap.Pr, uh, u + 4, fl);
 #pragma unroll
   for (int c = 0; c < NCU; ++c) {
-    T s = fl[c][0] * n[0];
+    TF s = fl[c][0] * n[0];
 #pragma unroll
     for (int d = 1; d < ND; ++d) s = s + fl[c][d] * n[d];
     if constexpr (C::PHYS == PHYS_NS) s = s + av * (u[4 + c] * n[0] + u[8 + c] * n[1]);
@@ -80,13 +84,14 @@ HDGX_DEV void face_fhat(const DevApp& ap, double av, const double* n, const T* u
 // an = A_n, An = |A_n| for NS and 3D Euler (ns/fbou.m:37-46,
 // fbou_euler3d.c:20-60); 2D Euler inviscid wall (ib 2: euler/fbou.m:30-48).
 // Host validation rejects other boundary types.
-template <class C, class T>
-HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const double* n, const T* u, const T* uh,
-                        T fh[C::NCU]) {
+template <class C, class TU, class TH>
+HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const double* n, const TU* u, const TH* uh,
+                        prom_t<TU, TH> fh[C::NCU]) {
   constexpr int ND = C::ND, NCU = C::NCU;
+  typedef prom_t<TU, TH> T;
   if (ib == 2) {
     if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
-      T un = u[1] * n[0] + u[2] * n[1];
+      TU un = u[1] * n[0] + u[2] * n[1];
       fh[0] = u[0] - uh[0];
       fh[1] = (u[1] - n[0] * un) - uh[1];
       fh[2] = (u[2] - n[1] * un) - uh[2];
@@ -94,28 +99,50 @@ HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const doub
     }
     return;
   }
-  T x[NCU], y[NCU];
   if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
     const double gam1 = ap.gam - 1.0;
-    T r1 = 1.0 / uh[0];
-    T uv = uh[1] * r1, vv = uh[2] * r1;
-    T af = 0.5 * (uv * uv + vv * vv);
-    T p = gam1 * (uh[3] - uh[0] * af);
-    T c = dsqrt(ap.gam * p * r1);
-    T un = uv * n[0] + vv * n[1];
-    T l2 = dabs(un) + c, l3 = un - c;
+    TH r1 = 1.0 / uh[0];
+    TH uv = uh[1] * r1, vv = uh[2] * r1;
+    TH af = 0.5 * (uv * uv + vv * vv);
+    TH p = gam1 * (uh[3] - uh[0] * af);
+    TH c = dsqrt(ap.gam * p * r1);
+    TH un = uv * n[0] + vv * n[1];
+    TH l2 = dabs(un) + c, l3 = un - c;
 #pragma unroll
     for (int k = 0; k < NCU; ++k) fh[k] = l2 * (u[k] - uinf[k]) + l3 * ((u[k] - uh[k]) + (uinf[k] - uh[k]));
   } else {
+    TU x0[NCU];
+    T x[NCU], y[NCU];
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) x[k] = u[k] - uinf[k];
-    roe_apply<ND>(ap.gam, ap.eps, 0, uh, n, x, y);
+    for (int k = 0; k < NCU; ++k) x0[k] = u[k] - uinf[k];
+    roe_apply<ND>(ap.gam, ap.eps, 0, uh, n, x0, y);
 #pragma unroll
     for (int k = 0; k < NCU; ++k) { fh[k] = y[k]; x[k] = (u[k] - uh[k]) + (uinf[k] - uh[k]); }
     roe_apply<ND>(ap.gam, ap.eps, 1, uh, n, x, y);
 #pragma unroll
     for (int k = 0; k < NCU; ++k) fh[k] = fh[k] + y[k];
   }
+}
+
+// one face point's fhat (fa) and boundary flux (fb: fbou on a boundary face,
+// else fhat), differentiated along whichever of u, uh is the dual
+template <class C, class TU, class TH>
+HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU* u, const TH* uh, int bf,
+                         D1 fa[C::NCU], D1 fb[C::NCU]) {
+  constexpr int NCU = C::NCU;
+  prom_t<TU, TH> fh[NCU];
+  face_fhat<C>(ap, av, nl, u, uh, fh);
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) fa[c] = fh[c];
+  if (bf < 0) {
+    const int id = -bf - 1;
+    double uinf[NCU];
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) uinf[c] = ap.bcs[id * 5 + c];
+    face_fbou<C>(ap, ap.bcm[id], uinf, nl, u, uh, fh);
+  }
+#pragma unroll
+  for (int c = 0; c < NCU; ++c) fb[c] = fh[c];
 }
 
 // LDS of one element's quadrature (one workgroup).
@@ -340,8 +367,10 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   __syncthreads();
 
   // ---- 2. physics, one task per (Gauss point, derivative direction)
-  for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
-    if (w < NGV * NC) {
+  // volume and face tasks in separate loops (their register needs do not add
+  // up); task w is thread w mod NT's as in one loop over both
+  for (int w = tid; w < NGV * NC; w += NT) {
+    {
       // volume point g, direction UDG component k (volint.m:30-254)
       const int g = w % NGV, k = w / NGV;
       D1 u[NC];
@@ -381,43 +410,57 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           }
         }
       }
-    } else {
-      // face point t, direction k: UDG component k (k < NC) or UH component
-      // k - NC (faceint.m:55-277)
-      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
-      const int j = t / NGF;
-      D1 u[NC], uh[NCU];
+    }
+  }
+  // face point t, direction k: UDG component k (k < NC, uh constant; tasks
+  // NGV*NC .. +NFG*NC) or UH component k - NC (u constant; the rest), each
+  // kind in a loop of its own (faceint.m:55-277)
+  auto face_out = [&](int t, int k, const D1* fa, const D1* fb) {
+    const double jac = sFJ[t];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) { u[c].v = sUf[t + NFG * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
-#pragma unroll
-      for (int c = 0; c < NCU; ++c) { uh[c].v = sUHf[t + NFG * c]; uh[c].d[0] = (c + NC == k) ? 1.0 : 0.0; }
-      double nl[ND];
-#pragma unroll
-      for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
-      const double jac = sFJ[t];
-      D1 fh[NCU];
-      face_fhat<C>(ap, sAvf[t], nl, u, uh, fh);
-#pragma unroll
-      for (int c = 0; c < NCU; ++c) {
-        if (k < NC) sFHu[t + NFG * (c + NCU * k)] = fh[c].d[0] * jac;
-        else sFHh[t + NFG * (c + NCU * (k - NC))] = fh[c].d[0] * jac;
-        if (k == 0) sFH[t + NFG * c] = fh[c].v * jac;
+    for (int c = 0; c < NCU; ++c) {
+      if (k < NC) {
+        sFHu[t + NFG * (c + NCU * k)] = fa[c].d[0] * jac;
+        sGHu[t + NFG * (c + NCU * k)] = fb[c].d[0] * jac;
+      } else {
+        sFHh[t + NFG * (c + NCU * (k - NC))] = fa[c].d[0] * jac;
+        sGHh[t + NFG * (c + NCU * (k - NC))] = fb[c].d[0] * jac;
       }
-      const int bfj = sBf[j];
-      if (bfj < 0) {
-        const int id = -bfj - 1;
-        double uinf[NCU];
-#pragma unroll
-        for (int c = 0; c < NCU; ++c) uinf[c] = ap.bcs[id * 5 + c];
-        face_fbou<C>(ap, ap.bcm[id], uinf, nl, u, uh, fh);
-      }
-#pragma unroll
-      for (int c = 0; c < NCU; ++c) {
-        if (k < NC) sGHu[t + NFG * (c + NCU * k)] = fh[c].d[0] * jac;
-        else sGHh[t + NFG * (c + NCU * (k - NC))] = fh[c].d[0] * jac;
-        if (k == 0) sGH[t + NFG * c] = fh[c].v * jac;
+      if (k == 0) {
+        sFH[t + NFG * c] = fa[c].v * jac;
+        sGH[t + NFG * c] = fb[c].v * jac;
       }
     }
+  };
+  for (int w = tid; w < NGV * NC + NFG * NC; w += NT) {
+    if (w < NGV * NC) continue;
+    const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+    double nl[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+    D1 u[NC], fa[NCU], fb[NCU];
+    double uh[NCU];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) { u[c].v = sUf[t + NFG * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) uh[c] = sUHf[t + NFG * c];
+    face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
+    face_out(t, k, fa, fb);
+  }
+  for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
+    if (w < NGV * NC + NFG * NC) continue;
+    const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+    double nl[ND];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+    double u[NC];
+    D1 uh[NCU], fa[NCU], fb[NCU];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) u[c] = sUf[t + NFG * c];
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) { uh[c].v = sUHf[t + NFG * c]; uh[c].d[0] = (c + NC == k) ? 1.0 : 0.0; }
+    face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
+    face_out(t, k, fa, fb);
   }
   __syncthreads();
 
@@ -588,7 +631,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 }
 
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+__global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C> lds;
   quad_body<C>(A, T, ap, blockIdx.x, lds);
 }
