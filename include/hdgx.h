@@ -85,6 +85,15 @@ int hdgx_create(hdgx_ctx** ctx, int device);
 int hdgx_destroy(hdgx_ctx* ctx);
 /* stream is a hipStream_t (NULL = default stream); calls are async on it */
 int hdgx_set_stream(hdgx_ctx* ctx, void* stream);
+/* Arithmetic of the local elimination of later hdgx_elemental / hdgx_schur_*
+ * calls on ctx (the reference is FP64 throughout, kernel/schur_uq.c:40-287):
+ *   HDGX_PREC_FP64     FP64 Gauss-Jordan on MFMA (default);
+ *   HDGX_PREC_FP32_IR  FP32 Gauss-Jordan and D'^-1 on MFMA, every solution
+ *                      column refined in FP64 against the element's FP64 BD
+ *                      until the predicted error is below 1e-13 relative
+ *                      (HDGX_ST_IR_SLOW in status[e] if 4 sweeps do not get
+ *                      there); AE, FE and the q rows stay FP64.
+ * The quadrature is FP64 in both. */
 int hdgx_set_precision(hdgx_ctx* ctx, int precision);
 const char* hdgx_last_error(hdgx_ctx* ctx);
 
