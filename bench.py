@@ -137,6 +137,25 @@ def cpu_baseline(P, nthreads):
     return out, P.ne / t_all, ns / t_one, kind_build, sum(runs) + sum(r1), floor
 
 
+def residual_leg(torch, ctx, P, dev, steps, warmup):
+    """The residual without Jacobians (hdgx_residual, kernel/hdg_residual.m) on
+    the same elements; Ru/Rh/Rq checked against the element-matrix kernel's
+    (same arithmetic)."""
+    from hdg_amd.runtime import Dev
+    npv, ncu, nd, ndf, ne = P.master.npv, P.app.ncu, P.master.nd, P.master.ndf, P.ne
+    rout = (Dev.empty((npv, ncu, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, ncu, nd, ne)))
+    ms, _ = time_steps(torch, ctx.stream, lambda: ctx.residual(P.master, P.app, *dev, out=rout), steps, warmup)
+    mats = ctx.elemental_matrices(P.master, P.app, *dev)
+    torch.cuda.synchronize()
+    err = max(relerr(rout[0].numpy(), mats["Ru"].numpy()), relerr(rout[1].numpy(), -mats["Rh"].numpy()),
+              relerr(rout[2].numpy(), mats["Rq"].numpy()))
+    del mats
+    log(f"[bench] residual: {ms:.3f} ms/step, {ne / ms * 1e3:.4g} elements/s")
+    return {"workload": "hdg_residual element loop (Ru, Rh, Rq; no Jacobians), same elements",
+            "ms_per_step": ms, "value": ne / (ms * 1e-3), "unit": "elements/s",
+            "max_rel_diff_vs_element_matrices": err}
+
+
 def serving_leg(nreq, seed):
     """Poisson stream of BDF1 Navier-Stokes Newton steps through the
     continuous-batching loop: p50/p99 latency and throughput."""
@@ -306,6 +325,7 @@ def main():
 
     tq, ts = kernel_times(torch, ctx, P, dev, out, reps=5)
     log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
+    resid = residual_leg(torch, ctx, P, dev, a.steps, a.warmup)
 
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ref, cpu_rate, cpu_rate1, build_kind, cpu_secs, noise_floor = cpu_baseline(P, ncpu)
@@ -356,6 +376,7 @@ def main():
                                    f"{build_kind} build, OpenMP over elements, median of 3 after 1 warm-up",
                          "single_core_value": cpu_rate1, "single_core_sample": f"first {min(2048, ne)} elements"},
         "speedup_vs_cpu": value / cpu_rate,
+        "residual": resid,
     }
     del out, dev
     torch.cuda.empty_cache()

@@ -142,6 +142,15 @@ int launch_quad(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const Quad
 }
 
 template <class C>
+int launch_resid(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const QuadArgs& qa) {
+  if (mb.npv != C::NPV || mb.npf != C::NPF || mb.ngv != C::NGV || mb.ngf != C::NGF)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled pgauss = 2p configuration");
+  if (qa.ne == 0) return HDGX_OK;
+  HIP_TRY(ctx, resid_launch<C>(ctx->stream, qa, mb.tab, ap));
+  return HDGX_OK;
+}
+
+template <class C>
 int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa, const int32_t* perm) {
   if (sa.ne == 0) return HDGX_OK;
   HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa, perm));
@@ -249,6 +258,28 @@ int hdgx_elemental_matrices(hdgx_ctx* ctx, const hdgx_master* master, const hdgx
   QuadArgs qa{ne, dgnodes, bf, udg, uh, sh, BD, F, GK, H, Ru, Rh, M, C, E, Rq, nullptr};
   const int p = porder_of(master->nd, master->npv);
   rc = with_cfg(app->physics, master->nd, p, [&](auto cfg) { return launch_quad<decltype(cfg)>(ctx, *mb, ap, qa); });
+  if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+  return rc;
+}
+
+int hdgx_residual(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne, const double* dgnodes,
+                  const int32_t* bf, const double* udg, const double* uh, const double* sh, double* Ru, double* Rh,
+                  double* Rq) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  DevApp ap;
+  if ((rc = make_app(ctx, master, app, &ap))) return rc;
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !Ru || !Rh)) return fail(ctx, HDGX_ERR_ARG, "null array");
+  if (ne > 0 && app->localsolve && !Rq) return fail(ctx, HDGX_ERR_ARG, "localsolve needs Rq");
+  if (ne > 0 && app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
+  QuadArgs qa{ne, dgnodes, bf, udg, uh, sh, nullptr, nullptr, nullptr, nullptr, Ru, Rh,
+              nullptr, nullptr, nullptr, Rq, nullptr};
+  const int p = porder_of(master->nd, master->npv);
+  rc = with_cfg(app->physics, master->nd, p, [&](auto cfg) { return launch_resid<decltype(cfg)>(ctx, *mb, ap, qa); });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
   return rc;
 }

@@ -16,6 +16,12 @@ hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, con
 }
 
 template <class C>
+hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
+  hipLaunchKernelGGL(resid_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+  return hipGetLastError();
+}
+
+template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
   if (a.prec == 1)
     hipLaunchKernelGGL((schur_mfma_ir_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
@@ -36,6 +42,7 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
 
 using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
 template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
+template hipError_t resid_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
 template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*);
 template hipError_t seg_launch<ThisCfg>(hipStream_t, const SegTable&, const DevMaster&, const int32_t*);
 

@@ -19,6 +19,8 @@
 namespace hdgx {
 
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
+// residual only (Ru, Rh with hdg_residual's sign, Rq)
+template <class C> hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
 // prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 MFMA Gauss-Jordan (with D'^-1) + FP64
 // refinement.
 // perm (device, may be null) marks GK as face sparse.
@@ -29,6 +31,8 @@ template <class C> hipError_t seg_launch(hipStream_t s, const SegTable& t, const
 #define HDGX_DECLARE(PH, ND, P)                                                                          \
   extern template hipError_t quad_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
                                                          const DevApp&);                                  \
+  extern template hipError_t resid_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
+                                                          const DevApp&);                                 \
   extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&, const int32_t*); \
   extern template hipError_t seg_launch<Cfg<PH, ND, P>>(hipStream_t, const SegTable&, const DevMaster&,  \
                                                         const int32_t*);

@@ -145,8 +145,9 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
   for (int c = 0; c < NCU; ++c) fb[c] = fh[c];
 }
 
-// LDS of one element's quadrature (one workgroup).
-template <class C>
+// LDS of one element's quadrature (one workgroup).  RO: residual only (no
+// Jacobian arrays).
+template <class C, bool RO = false>
 struct QuadLds {
   static constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   static constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NH = C::NH;
@@ -172,21 +173,25 @@ struct QuadLds {
   double sUf[NFG * NC];
   double sUHf[NFG * NCU];
   double sWrk[NR * NCU];
-  double sWrl[NR * NCU * NC];
+  double sWrl[RO ? 1 : NR * NCU * NC];
   double sFH[NFG * NCU];
-  double sFHu[NFG * NCU * NC];
-  double sFHh[NFG * NCU * NCU];
+  double sFHu[RO ? 1 : NFG * NCU * NC];
+  double sFHh[RO ? 1 : NFG * NCU * NCU];
   double sGH[NFG * NCU];
-  double sGHu[NFG * NCU * NC];
-  double sGHh[NFG * NCU * NCU];
+  double sGHu[RO ? 1 : NFG * NCU * NC];
+  double sGHh[RO ? 1 : NFG * NCU * NCU];
   double sM[UQ ? NPV * NPV : 1];
   double sC[UQ ? NPV * NPV * ND : 1];
   double sE[UQ ? NPV * NDF * ND : 1];
 };
 
 // Element e of the batch A (a workgroup of QUAD_THREADS threads), LDS in L.
-template <class C>
-HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e, QuadLds<C>& L) {
+// RO: the residual only (hdg_residual.m:1-57: Ru, Rq and Rh with that
+// function's sign, +shapfg (fbou | fhat), hdg_residual.m:286-290), no
+// Jacobians and no element matrices.
+template <class C, bool RO = false>
+HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e,
+                        QuadLds<C, RO>& L) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
   constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
@@ -366,144 +371,194 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   __syncthreads();
 
-  // ---- 2. physics, one task per (Gauss point, derivative direction)
-  // volume and face tasks in separate loops (their register needs do not add
-  // up); task w is thread w mod NT's as in one loop over both
-  for (int w = tid; w < NGV * NC; w += NT) {
-    {
-      // volume point g, direction UDG component k (volint.m:30-254)
-      const int g = w % NGV, k = w / NGV;
-      D1 u[NC];
+  if constexpr (RO) {
+    // ---- 2. physics values only: one task per volume Gauss point, then one
+    // per face point
+    for (int w = tid; w < NGV + NFG; w += NT) {
+      if (w < NGV) {
+        const int g = w;
+        double u[NC], f[NCU][ND], Xx[ND][ND];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) { u[c].v = sUg[g + NGV * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
-      D1 f[NCU][ND];
-      vol_flux<C>(ap, sAv[g], u, f);
-      const double jac = sJac[g];
-      double Xx[ND][ND];
+        for (int c = 0; c < NC; ++c) u[c] = sUg[g + NGV * c];
+        vol_flux<C>(ap, sAv[g], u, f);
+        const double jac = sJac[g];
 #pragma unroll
-      for (int a = 0; a < ND; ++a)
+        for (int a = 0; a < ND; ++a)
 #pragma unroll
-        for (int b = 0; b < ND; ++b) Xx[a][b] = sXx[g * ND * ND + a * ND + b];
-#pragma unroll
-      for (int c = 0; c < NCU; ++c) {
-        sWrl[g + NR * (c + NCU * k)] = (tdep && k == c) ? ap.fc_u * jac : 0.0;
-#pragma unroll
-        for (int i = 0; i < ND; ++i) {
-          double v = 0.0;
-#pragma unroll
-          for (int j = 0; j < ND; ++j) v += f[c][j].d[0] * Xx[j][i];
-          sWrl[g + NGV * (i + 1) + NR * (c + NCU * k)] = -v;
-        }
-      }
-      if (k == 0) {
-        // zero source (euler/source.m, ns/source.m) plus the BDF time term
+          for (int b = 0; b < ND; ++b) Xx[a][b] = sXx[g * ND * ND + a * ND + b];
 #pragma unroll
         for (int c = 0; c < NCU; ++c) {
-          const double s = tdep ? sSg[g + NGV * c] - u[c].v * ap.fc_u : 0.0;
-          sWrk[g + NR * c] = s * jac;
+          const double sv = tdep ? sSg[g + NGV * c] - u[c] * ap.fc_u : 0.0;
+          sWrk[g + NR * c] = sv * jac;
 #pragma unroll
           for (int i = 0; i < ND; ++i) {
             double v = 0.0;
 #pragma unroll
-            for (int j = 0; j < ND; ++j) v += f[c][j].v * Xx[j][i];
+            for (int j = 0; j < ND; ++j) v += f[c][j] * Xx[j][i];
             sWrk[g + NGV * (i + 1) + NR * c] = v;
           }
         }
-      }
-    }
-  }
-  // face point t, direction k: UDG component k (k < NC, uh constant; tasks
-  // NGV*NC .. +NFG*NC) or UH component k - NC (u constant; the rest), each
-  // kind in a loop of its own (faceint.m:55-277)
-  auto face_out = [&](int t, int k, const D1* fa, const D1* fb) {
-    const double jac = sFJ[t];
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) {
-      if (k < NC) {
-        sFHu[t + NFG * (c + NCU * k)] = fa[c].d[0] * jac;
-        sGHu[t + NFG * (c + NCU * k)] = fb[c].d[0] * jac;
       } else {
-        sFHh[t + NFG * (c + NCU * (k - NC))] = fa[c].d[0] * jac;
-        sGHh[t + NFG * (c + NCU * (k - NC))] = fb[c].d[0] * jac;
-      }
-      if (k == 0) {
-        sFH[t + NFG * c] = fa[c].v * jac;
-        sGH[t + NFG * c] = fb[c].v * jac;
+        const int t = w - NGV;
+        double nl[ND], u[NC], uh[NCU];
+#pragma unroll
+        for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+#pragma unroll
+        for (int c = 0; c < NC; ++c) u[c] = sUf[t + NFG * c];
+#pragma unroll
+        for (int c = 0; c < NCU; ++c) uh[c] = sUHf[t + NFG * c];
+        D1 fa[NCU], fb[NCU];
+        face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
+        const double jac = sFJ[t];
+#pragma unroll
+        for (int c = 0; c < NCU; ++c) {
+          sFH[t + NFG * c] = fa[c].v * jac;
+          sGH[t + NFG * c] = fb[c].v * jac;
+        }
       }
     }
-  };
-  for (int w = tid; w < NGV * NC + NFG * NC; w += NT) {
-    if (w < NGV * NC) continue;
-    const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
-    double nl[ND];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
-    D1 u[NC], fa[NCU], fb[NCU];
-    double uh[NCU];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) { u[c].v = sUf[t + NFG * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) uh[c] = sUHf[t + NFG * c];
-    face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
-    face_out(t, k, fa, fb);
-  }
-  for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
-    if (w < NGV * NC + NFG * NC) continue;
-    const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
-    double nl[ND];
-#pragma unroll
-    for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
-    double u[NC];
-    D1 uh[NCU], fa[NCU], fb[NCU];
-#pragma unroll
-    for (int c = 0; c < NC; ++c) u[c] = sUf[t + NFG * c];
-#pragma unroll
-    for (int c = 0; c < NCU; ++c) { uh[c].v = sUHf[t + NFG * c]; uh[c].d[0] = (c + NC == k) ? 1.0 : 0.0; }
-    face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
-    face_out(t, k, fa, fb);
-  }
-  __syncthreads();
-
-  // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
-  // faceint.m:316,326), one column (c, ii, k) per task
-  double* BD = A.BD + e * (int64_t)NU * C::NCOL;
-  for (int col = tid; col < NCU * NPV * NC; col += NT) {
-    const int c = col % NCU, ii = (col / NCU) % NPV, k = col / (NCU * NPV);
-    const double* wl = sWrl + NR * (c + NCU * k);
-    double acc[NPV];
-#pragma unroll
-    for (int jj = 0; jj < NPV; ++jj) acc[jj] = 0.0;
-    double sv[NGV];
-#pragma unroll
-    for (int g = 0; g < NGV; ++g) sv[g] = sShV[g + NGV * ii];
-#pragma unroll
-    for (int d = 0; d <= ND; ++d)
-#pragma unroll
-      for (int g = 0; g < NGV; ++g) {
-        const int r = g + NGV * d;
-        const double tv = sv[g] * wl[r];
-#pragma unroll
-        for (int jj = 0; jj < NPV; ++jj) acc[jj] += uload(T.shapvg, jj + NPV * r) * tv;
-      }
-#pragma unroll
-    for (int j = 0; j < NFE; ++j) {
-      const int lb = sLoc[ii + NPV * j];
-      if (lb >= 0) {
-        const double* fu = sFHu + j * NGF + NFG * (c + NCU * k);
-#pragma unroll
-        for (int jj = 0; jj < NPV; ++jj) {
-          const int la = uload(T.loc, jj + NPV * j);
-          if (la >= 0) {
+  } else {
+    // ---- 2. physics, one task per (Gauss point, derivative direction)
+    // volume and face tasks in separate loops (their register needs do not add
+    // up); task w is thread w mod NT's as in one loop over both
+    for (int w = tid; w < NGV * NC; w += NT) {
+      {
+        // volume point g, direction UDG component k (volint.m:30-254)
+        const int g = w % NGV, k = w / NGV;
+        D1 u[NC];
+  #pragma unroll
+        for (int c = 0; c < NC; ++c) { u[c].v = sUg[g + NGV * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
+        D1 f[NCU][ND];
+        vol_flux<C>(ap, sAv[g], u, f);
+        const double jac = sJac[g];
+        double Xx[ND][ND];
+  #pragma unroll
+        for (int a = 0; a < ND; ++a)
+  #pragma unroll
+          for (int b = 0; b < ND; ++b) Xx[a][b] = sXx[g * ND * ND + a * ND + b];
+  #pragma unroll
+        for (int c = 0; c < NCU; ++c) {
+          sWrl[g + NR * (c + NCU * k)] = (tdep && k == c) ? ap.fc_u * jac : 0.0;
+  #pragma unroll
+          for (int i = 0; i < ND; ++i) {
             double v = 0.0;
-#pragma unroll
-            for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * lb + NPF * NPF * g] * fu[g];
-            acc[jj] += v;
+  #pragma unroll
+            for (int j = 0; j < ND; ++j) v += f[c][j].d[0] * Xx[j][i];
+            sWrl[g + NGV * (i + 1) + NR * (c + NCU * k)] = -v;
+          }
+        }
+        if (k == 0) {
+          // zero source (euler/source.m, ns/source.m) plus the BDF time term
+  #pragma unroll
+          for (int c = 0; c < NCU; ++c) {
+            const double s = tdep ? sSg[g + NGV * c] - u[c].v * ap.fc_u : 0.0;
+            sWrk[g + NR * c] = s * jac;
+  #pragma unroll
+            for (int i = 0; i < ND; ++i) {
+              double v = 0.0;
+  #pragma unroll
+              for (int j = 0; j < ND; ++j) v += f[c][j].v * Xx[j][i];
+              sWrk[g + NGV * (i + 1) + NR * c] = v;
+            }
           }
         }
       }
     }
-#pragma unroll
-    for (int jj = 0; jj < NPV; ++jj) BD[jj + NPV * col] = acc[jj];
+    // face point t, direction k: UDG component k (k < NC, uh constant; tasks
+    // NGV*NC .. +NFG*NC) or UH component k - NC (u constant; the rest), each
+    // kind in a loop of its own (faceint.m:55-277)
+    auto face_out = [&](int t, int k, const D1* fa, const D1* fb) {
+      const double jac = sFJ[t];
+  #pragma unroll
+      for (int c = 0; c < NCU; ++c) {
+        if (k < NC) {
+          sFHu[t + NFG * (c + NCU * k)] = fa[c].d[0] * jac;
+          sGHu[t + NFG * (c + NCU * k)] = fb[c].d[0] * jac;
+        } else {
+          sFHh[t + NFG * (c + NCU * (k - NC))] = fa[c].d[0] * jac;
+          sGHh[t + NFG * (c + NCU * (k - NC))] = fb[c].d[0] * jac;
+        }
+        if (k == 0) {
+          sFH[t + NFG * c] = fa[c].v * jac;
+          sGH[t + NFG * c] = fb[c].v * jac;
+        }
+      }
+    };
+    for (int w = tid; w < NGV * NC + NFG * NC; w += NT) {
+      if (w < NGV * NC) continue;
+      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+      double nl[ND];
+  #pragma unroll
+      for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+      D1 u[NC], fa[NCU], fb[NCU];
+      double uh[NCU];
+  #pragma unroll
+      for (int c = 0; c < NC; ++c) { u[c].v = sUf[t + NFG * c]; u[c].d[0] = (c == k) ? 1.0 : 0.0; }
+  #pragma unroll
+      for (int c = 0; c < NCU; ++c) uh[c] = sUHf[t + NFG * c];
+      face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
+      face_out(t, k, fa, fb);
+    }
+    for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
+      if (w < NGV * NC + NFG * NC) continue;
+      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+      double nl[ND];
+  #pragma unroll
+      for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
+      double u[NC];
+      D1 uh[NCU], fa[NCU], fb[NCU];
+  #pragma unroll
+      for (int c = 0; c < NC; ++c) u[c] = sUf[t + NFG * c];
+  #pragma unroll
+      for (int c = 0; c < NCU; ++c) { uh[c].v = sUHf[t + NFG * c]; uh[c].d[0] = (c + NC == k) ? 1.0 : 0.0; }
+      face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
+      face_out(t, k, fa, fb);
+    }
+  }
+  __syncthreads();
+
+  if constexpr (!RO) {
+    // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
+    // faceint.m:316,326), one column (c, ii, k) per task
+    double* BD = A.BD + e * (int64_t)NU * C::NCOL;
+    for (int col = tid; col < NCU * NPV * NC; col += NT) {
+      const int c = col % NCU, ii = (col / NCU) % NPV, k = col / (NCU * NPV);
+      const double* wl = sWrl + NR * (c + NCU * k);
+      double acc[NPV];
+  #pragma unroll
+      for (int jj = 0; jj < NPV; ++jj) acc[jj] = 0.0;
+      double sv[NGV];
+  #pragma unroll
+      for (int g = 0; g < NGV; ++g) sv[g] = sShV[g + NGV * ii];
+  #pragma unroll
+      for (int d = 0; d <= ND; ++d)
+  #pragma unroll
+        for (int g = 0; g < NGV; ++g) {
+          const int r = g + NGV * d;
+          const double tv = sv[g] * wl[r];
+  #pragma unroll
+          for (int jj = 0; jj < NPV; ++jj) acc[jj] += uload(T.shapvg, jj + NPV * r) * tv;
+        }
+  #pragma unroll
+      for (int j = 0; j < NFE; ++j) {
+        const int lb = sLoc[ii + NPV * j];
+        if (lb >= 0) {
+          const double* fu = sFHu + j * NGF + NFG * (c + NCU * k);
+  #pragma unroll
+          for (int jj = 0; jj < NPV; ++jj) {
+            const int la = uload(T.loc, jj + NPV * j);
+            if (la >= 0) {
+              double v = 0.0;
+  #pragma unroll
+              for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * lb + NPF * NPF * g] * fu[g];
+              acc[jj] += v;
+            }
+          }
+        }
+      }
+  #pragma unroll
+      for (int jj = 0; jj < NPV; ++jj) BD[jj + NPV * col] = acc[jj];
+    }
   }
 
   // ---- 3b. Ru = shapvg*wrk - sum_faces shapfg*FH (volint.m:258, faceint.m:302-330)
@@ -529,54 +584,56 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     double v = 0.0;
 #pragma unroll
     for (int g = 0; g < NGF; ++g) v += T.shapfg[a + NPF * g] * sGH[j * NGF + g + NFG * c];
-    Rh[idx] = -v;
+    Rh[idx] = RO ? v : -v;
   }
-  // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327)
-  double* F = A.F + e * (int64_t)NU * NH;
-  for (int idx = tid; idx < NU * NH; idx += NT) {
-    const int va = idx % NPV;
-    int rest = idx / NPV;
-    const int c = rest % NCU;
-    rest /= NCU;
-    const int k = rest % NCU;
-    const int bj = rest / NCU, j = bj / NPF, b = bj - j * NPF;
-    const int la = sLoc[va + NPV * j];
-    double v = 0.0;
-    if (la >= 0)
-#pragma unroll
-      for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
-    F[idx] = v;
-  }
-  // ---- 3e. GK (ncu,ndf,npv,nc) (faceint.m:342-358)
-  double* GK = A.GK + e * (int64_t)NH * C::NCOL;
-  for (int idx = tid; idx < NH * C::NCOL; idx += NT) {
-    const int c = idx % NCU;
-    int rest = idx / NCU;
-    const int bj = rest % NDF;
-    rest /= NDF;
-    const int va = rest % NPV, k = rest / NPV;
-    const int j = bj / NPF, b = bj - j * NPF;
-    const int la = sLoc[va + NPV * j];
-    double v = 0.0;
-    if (la >= 0)
-#pragma unroll
-      for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
-    GK[idx] = v;
-  }
-  // ---- 3f. H (ncu,ndf,ncu,ndf), block diagonal per face (faceint.m:347-359)
-  double* H = A.H + e * (int64_t)NH * NH;
-  for (int idx = tid; idx < NH * NH; idx += NT) {
-    const int c = idx % NCU;
-    int rest = idx / NCU;
-    const int aj = rest % NDF;
-    rest /= NDF;
-    const int k = rest % NCU, bj = rest / NCU;
-    const int j = aj / NPF, a = aj - j * NPF, j2 = bj / NPF, b = bj - j2 * NPF;
-    double v = 0.0;
-    if (j == j2)
-#pragma unroll
-      for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
-    H[idx] = v;
+  if constexpr (!RO) {
+    // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327)
+    double* F = A.F + e * (int64_t)NU * NH;
+    for (int idx = tid; idx < NU * NH; idx += NT) {
+      const int va = idx % NPV;
+      int rest = idx / NPV;
+      const int c = rest % NCU;
+      rest /= NCU;
+      const int k = rest % NCU;
+      const int bj = rest / NCU, j = bj / NPF, b = bj - j * NPF;
+      const int la = sLoc[va + NPV * j];
+      double v = 0.0;
+      if (la >= 0)
+  #pragma unroll
+        for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
+      F[idx] = v;
+    }
+    // ---- 3e. GK (ncu,ndf,npv,nc) (faceint.m:342-358)
+    double* GK = A.GK + e * (int64_t)NH * C::NCOL;
+    for (int idx = tid; idx < NH * C::NCOL; idx += NT) {
+      const int c = idx % NCU;
+      int rest = idx / NCU;
+      const int bj = rest % NDF;
+      rest /= NDF;
+      const int va = rest % NPV, k = rest / NPV;
+      const int j = bj / NPF, b = bj - j * NPF;
+      const int la = sLoc[va + NPV * j];
+      double v = 0.0;
+      if (la >= 0)
+  #pragma unroll
+        for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
+      GK[idx] = v;
+    }
+    // ---- 3f. H (ncu,ndf,ncu,ndf), block diagonal per face (faceint.m:347-359)
+    double* H = A.H + e * (int64_t)NH * NH;
+    for (int idx = tid; idx < NH * NH; idx += NT) {
+      const int c = idx % NCU;
+      int rest = idx / NCU;
+      const int aj = rest % NDF;
+      rest /= NDF;
+      const int k = rest % NCU, bj = rest / NCU;
+      const int j = aj / NPF, a = aj - j * NPF, j2 = bj / NPF, b = bj - j2 * NPF;
+      double v = 0.0;
+      if (j == j2)
+  #pragma unroll
+        for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
+      H[idx] = v;
+    }
   }
   if constexpr (UQ) {
     // ---- 3g. M, C (volint.m:43-55), E (faceint.m:34-47)
@@ -586,7 +643,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
       v *= ap.fc_q;
       sM[idx] = v;
-      A.M[e * NPV * NPV + idx] = v;
+      if constexpr (!RO) A.M[e * NPV * NPV + idx] = v;
     }
     for (int idx = tid; idx < NPV * NPV * ND; idx += NT) {
       const int jjii = idx % (NPV * NPV), i = idx / (NPV * NPV);
@@ -596,7 +653,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 #pragma unroll
         for (int g = 0; g < NGV; ++g) v += T.svv[jjii + NPV * NPV * (g + NGV * (j + 1))] * sXx[g * ND * ND + i * ND + j];
       sC[idx] = v;
-      A.C[e * NPV * NPV * ND + idx] = v;
+      if constexpr (!RO) A.C[e * NPV * NPV * ND + idx] = v;
     }
     for (int idx = tid; idx < NPV * NDF * ND; idx += NT) {
       const int va = idx % NPV;
@@ -612,7 +669,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           v += sSff[la + NPF * b + NPF * NPF * g] * (sNl[t + NFG * i] * sFJ[t]);
         }
       sE[idx] = v;
-      A.E[e * NPV * NDF * ND + idx] = v;
+      if constexpr (!RO) A.E[e * NPV * NDF * ND + idx] = v;
     }
     __syncthreads();
     // ---- 3h. Rq = M (SH_q/fc_q - q) + C u - E uh (volint.m:57-60, faceint.m:48-51)
@@ -634,6 +691,13 @@ template <class C>
 __global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C> lds;
   quad_body<C>(A, T, ap, blockIdx.x, lds);
+}
+
+// residual only: Ru, Rh (hdg_residual.m's sign), Rq
+template <class C>
+__global__ __launch_bounds__(QUAD_THREADS) void resid_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+  __shared__ QuadLds<C, true> lds;
+  quad_body<C, true>(A, T, ap, blockIdx.x, lds);
 }
 
 }  // namespace hdgx

@@ -600,6 +600,31 @@ __global__ void gather_kernel(int ncu, int ndf, int64_t ne, const int32_t* __res
   }
 }
 
+// RH(c, m, i) = sum over the one or two elements of face i of their Rh at the
+// local face node that elcon maps to trace node i*npf + m (the sparse sum of
+// hdg_residual.m:41-52), one thread per trace unknown, no atomics
+__global__ void trace_scatter_kernel(int ncu, int npf, int nfe, int64_t nf, const int32_t* __restrict__ fconn,
+                                     const int32_t* __restrict__ elcon, const double* __restrict__ Rhe,
+                                     double* __restrict__ RH) {
+  const int64_t n = (int64_t)ncu * npf * nf;
+  const int ndf = npf * nfe;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)(t % ncu);
+    const int64_t node = t / ncu;               // i*npf + m
+    const int64_t i = node / npf;
+    double v = 0.0;
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      const int64_t e = fconn[2 * side + 4 * i];
+      if (e < 0) continue;
+      const int j = fconn[2 * side + 1 + 4 * i];
+      for (int a = 0; a < npf; ++a)
+        if (elcon[a + npf * j + (int64_t)ndf * e] == node) v += Rhe[c + ncu * (a + npf * j) + (int64_t)ncu * ndf * e];
+    }
+    RH[t] = v;
+  }
+}
+
 // DUDGT(:, e) = DUDG(:, e) + DUDG_DUH(:, :, e) DUH(:, elcon(:, e)), one element per workgroup
 __global__ __launch_bounds__(256) void local_update_kernel(int nr, int ncu, int ndf, int64_t ne,
                                                            const int32_t* __restrict__ elcon,
@@ -959,6 +984,20 @@ int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, cons
   const int64_t n = (int64_t)ncu * npf * nfe * ne;
   hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, ctx->stream, ncu, npf * nfe, ne, elcon,
                      UH, UHE);
+  HIP_TRY(ctx, hipGetLastError());
+  return HDGX_OK;
+}
+
+int hdgx_trace_scatter(hdgx_ctx* ctx, const hdgx_trace* T, const double* Rhe, double* RH) {
+  if (!ctx) return HDGX_ERR_ARG;
+  int rc = check_trace(ctx, T);
+  if (rc) return rc;
+  if (T->nf == 0) return HDGX_OK;
+  if (!T->elcon || !Rhe || !RH) return fail(ctx, HDGX_ERR_ARG, "trace_scatter: null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  const int64_t n = (int64_t)T->ncu * T->npf * T->nf;
+  hipLaunchKernelGGL(trace_scatter_kernel, dim3(grid_for(n, 256, 65536)), dim3(256), 0, ctx->stream, T->ncu, T->npf,
+                     T->nfe, T->nf, T->fconn, T->elcon, Rhe, RH);
   HIP_TRY(ctx, hipGetLastError());
   return HDGX_OK;
 }

@@ -45,6 +45,7 @@ def lib():
         L.hdgx_last_error.restype = ctypes.c_char_p
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
         L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
+        L.hdgx_residual.argtypes = [vp, vp, vp, i64] + [vp] * 8
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
         L.hdgx_elemental_batch.argtypes = [vp, vp, i32, vp]
@@ -58,6 +59,7 @@ def lib():
         L.hdgx_bilu0_apply.argtypes = [vp] * 5
         L.hdgx_gmres.argtypes = [vp] * 6 + [i32, ctypes.c_double, i32, vp, vp, vp]
         L.hdgx_trace_gather.argtypes = [vp, i32, i32, i32, i64, vp, vp, vp]
+        L.hdgx_trace_scatter.argtypes = [vp, vp, vp, vp]
         L.hdgx_local_update.argtypes = [vp, i32, i32, i32, i32, i32, i64, vp, vp, vp, vp, vp]
         L.hdgx_newton_update.argtypes = [vp, i32, i32, i32, i64, i64, ctypes.c_double, vp, vp, vp, vp, vp, vp]
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
@@ -66,10 +68,11 @@ def lib():
 
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
-           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_schur_u", "hdgx_schur_uq", "hdgx_elemental_batch",
+           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_schur_u", "hdgx_schur_uq",
+           "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
-           "hdgx_trace_gather", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
+           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
 
 
 def _torch():
@@ -261,6 +264,25 @@ class Context:
         self._check(lib().hdgx_elemental_matrices(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr,
                                                   u.ptr, h.ptr, _p(s), *ptrs))
         return out
+
+    def residual(self, master, app, dgnodes, bf, udg, uh, sh=None, out=None):
+        """(Ru, Rh, Rq) of hdg_residual's element loop (kernel/hdg_residual.m:1-37):
+        the residual without Jacobians; Rh per element with hdg_residual's sign
+        (TraceSystem.scatter assembles it); Rq None when localsolve = 0."""
+        ca = app.to_c() if isinstance(app, App) else app
+        mc = self.master_c(master)
+        dg, b, u, h = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh)
+        s = _dev(sh)
+        ne = dg.shape[2]
+        npv, ncu, nd = master.npv, ca.ncu, master.nd
+        ndf = master.npf * master.nfe
+        if out is None:
+            out = (Dev.empty((npv, ncu, ne)), Dev.empty((ncu, ndf, ne)),
+                   Dev.empty((npv, ncu, nd, ne)) if ca.localsolve else None)
+        Ru, Rh, Rq = out
+        self._check(lib().hdgx_residual(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr, h.ptr,
+                                        _p(s), Ru.ptr, Rh.ptr, _p(Rq)))
+        return Ru, Rh, Rq
 
     def _schur_out(self, npv, ncu, nc, ndf, ne):
         return (Dev.empty((npv, nc, ne)), Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ncu, ndf, ncu, ndf, ne)),

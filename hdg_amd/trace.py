@@ -182,6 +182,13 @@ class TraceSystem:
                                             UHE.ptr))
         return UHE
 
+    def scatter(self, Rhe, RH=None):
+        """RH (ncf, nf) = the per-element Rh (ncu, npf*nfe, ne) summed at the
+        global trace unknowns (kernel/hdg_residual.m:41-52)."""
+        RH = RH or Dev.empty((self.ncf, self.nf))
+        self._check(lib().hdgx_trace_scatter(self.ctx.h, ctypes.byref(self.c), Rhe.ptr, RH.ptr))
+        return RH
+
     def local_update(self, npv, nco, DUDG, DUDG_DUH, DUH, DUDGT=None):
         DUDGT = DUDGT or Dev.empty((npv, nco, self.ne))
         self._check(lib().hdgx_local_update(self.ctx.h, npv, nco, self.ncu, self.npf, self.nfe, self.ne,

@@ -123,6 +123,18 @@ int hdgx_elemental_matrices(hdgx_ctx* ctx, const hdgx_master* master, const hdgx
                             double* F, double* GK, double* H, double* Ru, double* Rh, double* M,
                             double* C, double* E, double* Rq);
 
+/* Residual only, no Jacobians.  Replaces the element part of
+ *   [RU,RH,RQ] = hdg_residual(master,app,mesh,UDG,UH,SH)  (kernel/hdg_residual.m:1-37)
+ * with its volint / faceint (hdg_residual.m:57-290): Ru (npv,ncu,ne) and Rq
+ * (npv,ncu,nd,ne; NULL when localsolve=0) as hdgx_elemental_matrices gives them,
+ * Rh (ncu,ndf,ne) per element with hdg_residual's sign, +shapfg (fbou | fhat)
+ * (hdg_residual.m:286-290; kernel/faceint.m:339 negates it).  The global RH of
+ * hdg_residual.m:41-52 is hdgx_trace_scatter of this Rh.  Inputs as
+ * hdgx_elemental; uh is the element-gathered trace (hdgx_trace_gather). */
+int hdgx_residual(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                  const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
+                  const double* sh, double* Ru, double* Rh, double* Rq);
+
 /* u-only Schur elimination.  Replaces the MEX
  *   [DUDG,DUDG_DUH,AE,FE] = schur_u(BD,F,GK,H,Ru,Rh)   (kernel/schur_u.c:82-163)
  * with the same argument order; inputs are const (the MEX clobbers BD and Ru). */
@@ -250,6 +262,12 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
  *   UH (ncu, nsiz), UHE (ncu, npf*nfe, ne). */
 int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, const int32_t* elcon,
                       const double* UH, double* UHE);
+
+/* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace
+ * residual of kernel/hdg_residual.m:41-52 (sparse(il, 1, RH)), summed per face
+ * from its one or two elements (no atomics, the same bits every run).
+ *   Rh (ncu, npf*nfe, ne), RH (ncu, npf*nf) in the layout of UH. */
+int hdgx_trace_scatter(hdgx_ctx* ctx, const hdgx_trace* T, const double* Rh, double* RH);
 
 /* Local update DUDGT = DUDG + DUDG_DUH * DUH(:, elcon) (kernel/hdg_solve.m:119-121,
  * util/mapContractK.m:47-99).  DUDG (npv,nco,ne), DUDG_DUH (npv,nco,ncu,ndf,ne),

@@ -60,6 +60,23 @@ def test_element_matrices(ctx, oracle, case):
 
 
 @pytest.mark.parametrize("case", list(CASES))
+def test_residual(ctx, oracle, case):
+    """hdgx_residual: Ru and Rq as the element matrices' and Rh with
+    hdg_residual.m's sign (the negative of faceint.m's)."""
+    P = _inputs(case)
+    print(f"[gpu] residual {case} ne={P.ne}", flush=True)
+    ref = oracle.elemental_matrices(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    Ru, Rh, Rq = ctx.residual(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    ctx.stream.synchronize()
+    assert relerr(Ru.numpy(), ref["Ru"]) < 1e-12
+    assert relerr(Rh.numpy(), -ref["Rh"]) < 1e-12
+    if P.app.localsolve:
+        assert relerr(Rq.numpy(), ref["Rq"]) < 1e-12
+    else:
+        assert Rq is None
+
+
+@pytest.mark.parametrize("case", list(CASES))
 def test_schur_boundary(ctx, oracle, case):
     """hdgx_schur_u / hdgx_schur_uq on the oracle's element matrices."""
     P = _inputs(case)

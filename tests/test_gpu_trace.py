@@ -154,6 +154,26 @@ def test_bilu0_factor_and_apply(ctx, oracle, case, ordering):
 
 
 @pytest.mark.parametrize("case", ["euler2d_p2", "ns2d_p3", "euler3d_p1"])
+def test_trace_scatter(ctx, oracle, case):
+    """hdgx_trace_scatter: the per-element Rh summed at the global trace
+    unknowns (hdg_residual.m:41-52), against a numpy scatter-add over elcon.
+    Every trace unknown has one or two terms, so the sums agree to the bit."""
+    from hdg_amd.runtime import Dev
+    from hdg_amd.trace import TraceSystem
+    P = CASES[case]()
+    print(f"[gpu] trace scatter {case}", flush=True)
+    ts = TraceSystem(ctx, P.mesh, P.app.ncu, with_bilu0=False)
+    ndf = ts.npf * ts.nfe
+    Rhe = np.asfortranarray(np.random.default_rng(5).standard_normal((ts.ncu, ndf, ts.ne)))
+    RH = ts.scatter(Dev.from_numpy(Rhe)).numpy().reshape(ts.ncu, ts.npf * ts.nf, order="F")
+    ref = np.zeros((ts.ncu, ts.npf * ts.nf))
+    for e in range(ts.ne):
+        for a in range(ndf):
+            ref[:, P.mesh.elcon[a, e]] += Rhe[:, a, e]
+    assert np.array_equal(RH, ref)
+
+
+@pytest.mark.parametrize("case", ["euler2d_p2", "ns2d_p3", "euler3d_p1"])
 def test_gather_local_and_newton_update(ctx, oracle, case):
     from hdg_amd.runtime import Dev
     from hdg_amd.trace import TraceSystem
