@@ -8,6 +8,7 @@
 #include <memory>
 #include <mutex>
 #include <algorithm>
+#include <cmath>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -151,6 +152,15 @@ int launch_resid(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const Qua
 }
 
 template <class C>
+int launch_getq(hdgx_ctx* ctx, const MasterBuf& mb, const GetqArgs& ga) {
+  if (mb.npv != C::NPV || mb.npf != C::NPF || mb.ngv != C::NGV || mb.ngf != C::NGF)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled pgauss = 2p configuration");
+  if (ga.ne == 0) return HDGX_OK;
+  HIP_TRY(ctx, getq_launch<C>(ctx->stream, ga, mb.tab));
+  return HDGX_OK;
+}
+
+template <class C>
 int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa, const int32_t* perm) {
   if (sa.ne == 0) return HDGX_OK;
   HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa, perm));
@@ -281,6 +291,32 @@ int hdgx_residual(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app,
   const int p = porder_of(master->nd, master->npv);
   rc = with_cfg(app->physics, master->nd, p, [&](auto cfg) { return launch_resid<decltype(cfg)>(ctx, *mb, ap, qa); });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+  return rc;
+}
+
+int hdgx_getq(hdgx_ctx* ctx, const hdgx_master* master, int ncu, int nc, int ncd, int64_t ne,
+              const double* dgnodes, const double* udg, const double* uh, const double* sh, double fc_q,
+              double* qdg, int64_t ldq, const double* shapnv, double* qq) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (!master) return fail(ctx, HDGX_ERR_ARG, "master: null");
+  const int nd = master->nd, ncq = ncu * nd;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (ncu != nd + 2) return fail(ctx, HDGX_ERR_UNSUPPORTED, "getq: ncu must be nd+2");
+  if (nc < ncu) return fail(ctx, HDGX_ERR_ARG, "getq: nc < ncu");
+  if (sh && nc < ncq) return fail(ctx, HDGX_ERR_ARG, "getq: sh needs nc >= ncu*nd");
+  if (ncd < nd) return fail(ctx, HDGX_ERR_ARG, "getq: ncd < nd");
+  if (ldq < (int64_t)master->npv * ncq) return fail(ctx, HDGX_ERR_ARG, "getq: ldq < npv*ncu*nd");
+  if (!(fc_q != 0.0) || !std::isfinite(fc_q)) return fail(ctx, HDGX_ERR_ARG, "getq: fc_q must be finite and nonzero");
+  if ((shapnv == nullptr) != (qq == nullptr)) return fail(ctx, HDGX_ERR_ARG, "getq: shapnv and qq go together");
+  if (ne > 0 && (!dgnodes || !udg || !uh || !qdg)) return fail(ctx, HDGX_ERR_ARG, "null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  GetqArgs ga{ne, nc, ncd, fc_q, dgnodes, udg, uh, sh, qdg, ldq, shapnv, qq};
+  const int p = porder_of(nd, master->npv);
+  rc = with_cfg(HDGX_PHYS_EULER, nd, p, [&](auto cfg) { return launch_getq<decltype(cfg)>(ctx, *mb, ga); });
+  if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "getq: no kernel for this (nd, p)");
   return rc;
 }
 

@@ -4,6 +4,7 @@
 
 #include "hdgx_launch.h"
 #include "quad_kernel.hip"
+#include "getq_kernel.hip"
 #include "schur_mfma.hip"
 #include "seg_kernels.hip"
 
@@ -18,6 +19,12 @@ hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, con
 template <class C>
 hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
   hipLaunchKernelGGL(resid_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+  return hipGetLastError();
+}
+
+template <class C>
+hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
+  hipLaunchKernelGGL(getq_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t);
   return hipGetLastError();
 }
 
@@ -43,6 +50,9 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
 using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
 template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
 template hipError_t resid_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
+#if HDGX_PHYS == 0
+template hipError_t getq_launch<ThisCfg>(hipStream_t, const GetqArgs&, const DevMaster&);
+#endif
 template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*);
 template hipError_t seg_launch<ThisCfg>(hipStream_t, const SegTable&, const DevMaster&, const int32_t*);
 

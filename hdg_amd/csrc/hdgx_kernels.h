@@ -64,6 +64,21 @@ struct QuadArgs {
   int32_t* status;
 };
 
+// q recovery (getq): per element q = (fc_q M)^-1 (C u - E uh) + SH_q/fc_q.
+struct GetqArgs {
+  int64_t ne;
+  int32_t nc, ncd;                 // columns of udg / sh and of dgnodes
+  double fc_q;
+  const double* dgnodes;           // (npv, ncd, ne)
+  const double* udg;               // (npv, nc, ne): the first ncu columns are read
+  const double* uh;                // (ncu, npf*nfe, ne)
+  const double* sh;                // (npv, nc, ne) or null: its last ncu*nd columns are added / fc_q
+  double* qdg;                     // element e at qdg + e*ldq: (npv, ncu*nd)
+  int64_t ldq;
+  const double* shapnv;            // (npv, npv, nd+1) nodal basis at the nodes, or null
+  double* qq;                      // (npv, ncu*nd, ne) or null
+};
+
 struct SchurArgs {
   int64_t ne;
   const double *M, *C, *E, *BD, *F, *GK, *H, *Rq, *Ru, *Rh;

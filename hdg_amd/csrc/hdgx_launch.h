@@ -21,6 +21,9 @@ namespace hdgx {
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
 // residual only (Ru, Rh with hdg_residual's sign, Rq)
 template <class C> hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
+// q recovery; geometry only, so it is built for the Euler configurations
+// (PH = 0), whose (nd, p) cover every built order
+template <class C> hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t);
 // prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 MFMA Gauss-Jordan (with D'^-1) + FP64
 // refinement.
 // perm (device, may be null) marks GK as face sparse.
@@ -38,6 +41,10 @@ template <class C> hipError_t seg_launch(hipStream_t s, const SegTable& t, const
                                                         const int32_t*);
 HDGX_FOR_EACH_CFG(HDGX_DECLARE)
 #undef HDGX_DECLARE
+#define HDGX_DECLARE_GETQ(PH, ND, P) \
+  extern template hipError_t getq_launch<Cfg<PH, ND, P>>(hipStream_t, const GetqArgs&, const DevMaster&);
+HDGX_FOR_EACH_CFG(HDGX_DECLARE_GETQ)
+#undef HDGX_DECLARE_GETQ
 
 // one persistent launch per kernel over the elements of every order
 template <int PH, int ND> hipError_t mixed_launch(hipStream_t s, const MixedTable* t, int64_t total, int grid);

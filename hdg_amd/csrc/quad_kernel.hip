@@ -43,6 +43,82 @@ HDGX_DEV int32_t uload(const int32_t* p, int i) { return ((ConstI)p)[i]; }
 
 // f (NCU x ND) at one point (euler/flux.m, ns/flux2d.m; laplacian AV f += av*q
 // as in nsshock/flux.m:16-21)
+// Jacobian determinant and adjugate of J[a][b] = dx_b/dxi_a: Xx = det(J) J^-1,
+// Xx[a][b] = jac * dxi_b/dx_a (kernel/volgeom.m:22-46).
+template <int ND>
+HDGX_DEV double adjugate(const double J[ND][ND], double Xx[ND][ND]) {
+  if constexpr (ND == 2) {
+    Xx[0][0] = J[1][1]; Xx[1][0] = -J[1][0]; Xx[0][1] = -J[0][1]; Xx[1][1] = J[0][0];
+    return J[0][0] * J[1][1] - J[0][1] * J[1][0];
+  } else {
+    Xx[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+    Xx[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+    Xx[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+    Xx[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
+    Xx[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
+    Xx[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
+    Xx[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
+    Xx[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
+    Xx[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+    return J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[2][1] * J[1][2] + J[1][0] * J[2][1] * J[0][2] -
+           J[1][0] * J[0][1] * J[2][2] + J[2][0] * J[0][1] * J[1][2] - J[2][0] * J[1][1] * J[0][2];
+  }
+}
+
+// Unit normal and surface Jacobian from the face tangents dpg[d][k] =
+// dx_d/ds_k (kernel/facegeom.m:13-52).
+template <int ND>
+HDGX_DEV double face_normal(const double dpg[ND][ND > 1 ? ND - 1 : 1], double nl[ND]) {
+  if constexpr (ND == 2) {
+    const double jac = sqrt(dpg[0][0] * dpg[0][0] + dpg[1][0] * dpg[1][0]);
+    nl[0] = dpg[1][0] / jac;
+    nl[1] = -dpg[0][0] / jac;
+    return jac;
+  } else {
+    const double n0 = dpg[1][0] * dpg[2][1] - dpg[2][0] * dpg[1][1];
+    const double n1 = dpg[2][0] * dpg[0][1] - dpg[0][0] * dpg[2][1];
+    const double n2 = dpg[0][0] * dpg[1][1] - dpg[1][0] * dpg[0][1];
+    const double jac = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
+    nl[0] = n0 / jac; nl[1] = n1 / jac; nl[2] = n2 / jac;
+    return jac;
+  }
+}
+
+// Volume Gauss point g: jac and Xx from the geometry basis shapmv and the
+// element nodes X (npv, ncd), column-major with leading dimension NPV.
+template <class C>
+HDGX_DEV double vol_geom(const DevMaster& T, const double* X, int g, double Xx[C::ND][C::ND]) {
+  constexpr int ND = C::ND, NPV = C::NPV, NGV = C::NGV;
+  double J[ND][ND];
+#pragma unroll
+  for (int j = 0; j < ND; ++j)
+#pragma unroll
+    for (int i = 0; i < ND; ++i) {
+      double v = 0.0;
+#pragma unroll
+      for (int a = 0; a < NPV; ++a) v += T.shapmv[g + NGV * a + NGV * NPV * (1 + j)] * X[a + NPV * i];
+      J[j][i] = v;   // dx_i / dxi_j
+    }
+  return adjugate<ND>(J, Xx);
+}
+
+// Face j, Gauss point g: unit normal and surface Jacobian.
+template <class C>
+HDGX_DEV double face_geom(const DevMaster& T, const double* X, int j, int g, double nl[C::ND]) {
+  constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NGF = C::NGF;
+  double dpg[ND][ND > 1 ? ND - 1 : 1];
+#pragma unroll
+  for (int d = 0; d < ND; ++d)
+#pragma unroll
+    for (int k = 0; k < ND - 1; ++k) {
+      double v = 0.0;
+#pragma unroll
+      for (int a = 0; a < NPF; ++a) v += T.shapmf[g + NGF * a + NGF * NPF * (1 + k)] * X[T.permgeom[a + NPF * j] + NPV * d];
+      dpg[d][k] = v;
+    }
+  return face_normal<ND>(dpg, nl);
+}
+
 template <class C, class T>
 HDGX_DEV void vol_flux(const DevApp& ap, double av, const T* u, T f[C::NCU][C::ND]) {
   if constexpr (C::PHYS == PHYS_EULER) {
@@ -268,35 +344,10 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     const int W5 = W4 + (tdep ? NGV * NCU : 0);
     for (int w = tid; w < W5; w += NT) {
       if (w < W0) {
-        // volume geometry: J, jac = det J, Xx(a, b) = jac * dxi_b/dx_a
+        // volume geometry: jac = det J, Xx(a, b) = jac * dxi_b/dx_a
         const int g = w;
-        double J[ND][ND];
-#pragma unroll
-        for (int j = 0; j < ND; ++j)
-#pragma unroll
-          for (int i = 0; i < ND; ++i) {
-            double v = 0.0;
-#pragma unroll
-            for (int a = 0; a < NPV; ++a) v += T.shapmv[g + NGV * a + NGV * NPV * (1 + j)] * sX[a + NPV * i];
-            J[j][i] = v;   // dx_i / dxi_j
-          }
-        double jac, Xx[ND][ND];
-        if constexpr (ND == 2) {
-          jac = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-          Xx[0][0] = J[1][1]; Xx[1][0] = -J[1][0]; Xx[0][1] = -J[0][1]; Xx[1][1] = J[0][0];
-        } else {
-          jac = J[0][0] * J[1][1] * J[2][2] - J[0][0] * J[2][1] * J[1][2] + J[1][0] * J[2][1] * J[0][2] -
-                J[1][0] * J[0][1] * J[2][2] + J[2][0] * J[0][1] * J[1][2] - J[2][0] * J[1][1] * J[0][2];
-          Xx[0][0] = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-          Xx[1][0] = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-          Xx[2][0] = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-          Xx[0][1] = J[0][2] * J[2][1] - J[0][1] * J[2][2];
-          Xx[1][1] = J[0][0] * J[2][2] - J[0][2] * J[2][0];
-          Xx[2][1] = J[0][1] * J[2][0] - J[0][0] * J[2][1];
-          Xx[0][2] = J[0][1] * J[1][2] - J[0][2] * J[1][1];
-          Xx[1][2] = J[0][2] * J[1][0] - J[0][0] * J[1][2];
-          Xx[2][2] = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-        }
+        double Xx[ND][ND];
+        const double jac = vol_geom<C>(T, sX, g, Xx);
         sJac[g] = jac;
 #pragma unroll
         for (int a = 0; a < ND; ++a)
@@ -310,29 +361,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         // face geometry: unit normal and surface Jacobian
         const int t = w - W0;
         const int j = t / NGF, g = t - j * NGF;
-        double dpg[ND][ND > 1 ? ND - 1 : 1];
-#pragma unroll
-        for (int d = 0; d < ND; ++d)
-#pragma unroll
-          for (int k = 0; k < ND - 1; ++k) {
-            double v = 0.0;
-#pragma unroll
-            for (int a = 0; a < NPF; ++a)
-              v += T.shapmf[g + NGF * a + NGF * NPF * (1 + k)] * sX[T.permgeom[a + NPF * j] + NPV * d];
-            dpg[d][k] = v;
-          }
-        double nl[ND], jac;
-        if constexpr (ND == 2) {
-          jac = sqrt(dpg[0][0] * dpg[0][0] + dpg[1][0] * dpg[1][0]);
-          nl[0] = dpg[1][0] / jac;
-          nl[1] = -dpg[0][0] / jac;
-        } else {
-          double n0 = dpg[1][0] * dpg[2][1] - dpg[2][0] * dpg[1][1];
-          double n1 = dpg[2][0] * dpg[0][1] - dpg[0][0] * dpg[2][1];
-          double n2 = dpg[0][0] * dpg[1][1] - dpg[1][0] * dpg[0][1];
-          jac = sqrt(n0 * n0 + n1 * n1 + n2 * n2);
-          nl[0] = n0 / jac; nl[1] = n1 / jac; nl[2] = n2 / jac;
-        }
+        double nl[ND];
+        const double jac = face_geom<C>(T, sX, j, g, nl);
 #pragma unroll
         for (int i = 0; i < ND; ++i) sNl[t + NFG * i] = nl[i];
         sFJ[t] = jac;

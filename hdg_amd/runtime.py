@@ -46,6 +46,7 @@ def lib():
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
         L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
         L.hdgx_residual.argtypes = [vp, vp, vp, i64] + [vp] * 8
+        L.hdgx_getq.argtypes = [vp, vp, i32, i32, i32, i64] + [vp] * 4 + [ctypes.c_double, vp, i64, vp, vp]
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
         L.hdgx_elemental_batch.argtypes = [vp, vp, i32, vp]
@@ -68,7 +69,7 @@ def lib():
 
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
-           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_schur_u", "hdgx_schur_uq",
+           "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
            "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
@@ -283,6 +284,34 @@ class Context:
         self._check(lib().hdgx_residual(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr, h.ptr,
                                         _p(s), Ru.ptr, Rh.ptr, _p(Rq)))
         return Ru, Rh, Rq
+
+    def getq(self, master, dgnodes, udg, uh, sh=None, fc_q=1.0, ncu=None, out=None, inplace=False, shapnv=None):
+        """QDG = getq(master, mesh, UDG, UH, SH, fc_q) (kernel/getq.m:1-129) for the
+        element-gathered trace uh (ncu, npf*nfe, ne).  inplace=True writes it over
+        UDG's q columns (hdg_solve.m:130-131; udg must then be a Dev) and returns
+        udg; otherwise returns QDG (npv, ncu*nd, ne).  With shapnv =
+        mkshape(p, plocvl, plocvl) it returns (QDG, qq)."""
+        nd, npv = master.nd, master.npv
+        ncu = ncu or nd + 2
+        ncq = ncu * nd
+        mc = self.master_c(master)
+        if inplace and not isinstance(udg, Dev):
+            raise HdgxError("getq(inplace=True) writes into udg: pass a device array")
+        dg, u, h, s = _dev(dgnodes), _dev(udg), _dev(uh), _dev(sh)
+        ne, nc, ncd = u.shape[2], u.shape[1], dg.shape[1]
+        if inplace:
+            if nc != ncu + ncq:
+                raise HdgxError("getq(inplace=True) needs udg with nc = ncu*(nd+1) columns")
+            qptr = ctypes.c_void_p(u.t.data_ptr() + 8 * npv * ncu)
+            ldq, res = npv * nc, u
+        else:
+            res = out if out is not None else Dev.empty((npv, ncq, ne))
+            qptr, ldq = res.ptr, npv * ncq
+        sn = _dev(shapnv)
+        qq = Dev.empty((npv, ncq, ne)) if sn is not None else None
+        self._check(lib().hdgx_getq(self.h, ctypes.byref(mc.c), ncu, nc, ncd, ne, dg.ptr, u.ptr, h.ptr, _p(s),
+                                    float(fc_q), qptr, ldq, _p(sn), _p(qq)))
+        return (res, qq) if qq is not None else res
 
     def _schur_out(self, npv, ncu, nc, ndf, ne):
         return (Dev.empty((npv, nc, ne)), Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ncu, ndf, ncu, ndf, ne)),

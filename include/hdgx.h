@@ -135,6 +135,25 @@ int hdgx_residual(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app,
                   const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
                   const double* sh, double* Ru, double* Rh, double* Rq);
 
+/* q recovery.  Replaces
+ *   [QDG, qq] = getq(master, mesh, UDG, UH, SH, fc_q)     (kernel/getq.m:1-129)
+ * as kernel/hdg_solve.m:130,149 and kernel/hdg_solve_dirk.m:89 call it:
+ *   QDG = (fc_q M)^-1 (C u - E uh) + SH(:, end-ncu*nd+1:end, :)/fc_q
+ * per element, the q that zeroes Rq of volint.m:57-60 / faceint.m:48-51.
+ *   dgnodes (npv,ncd,ne); udg (npv,nc,ne), columns 0..ncu-1 read; uh the
+ *   element-gathered trace (ncu,npf*nfe,ne) (hdgx_trace_gather); sh
+ *   (npv,nc,ne) or NULL.  Element e's (npv, ncu*nd) block of QDG is written
+ *   at qdg + e*ldq: ldq = npv*ncu*nd for a separate array, or qdg = udg +
+ *   npv*ncu with ldq = npv*nc to overwrite UDG(:, ncu+1:end, :) in place
+ *   (hdg_solve.m:131).  shapnv (npv,npv,nd+1) = mkshape(porder, plocvl,
+ *   plocvl) and qq (npv,ncu*nd,ne), the nodal gradient -grad u of
+ *   getq.m:62-76, are both given or both NULL.  fc_q must be nonzero.
+ * getq.m's MiCE cache (Mi, C, E per element) has no counterpart: the kernel
+ * rebuilds the geometry, which costs less than reading those matrices. */
+int hdgx_getq(hdgx_ctx* ctx, const hdgx_master* master, int ncu, int nc, int ncd, int64_t ne,
+              const double* dgnodes, const double* udg, const double* uh, const double* sh,
+              double fc_q, double* qdg, int64_t ldq, const double* shapnv, double* qq);
+
 /* u-only Schur elimination.  Replaces the MEX
  *   [DUDG,DUDG_DUH,AE,FE] = schur_u(BD,F,GK,H,Ru,Rh)   (kernel/schur_u.c:82-163)
  * with the same argument order; inputs are const (the MEX clobbers BD and Ru). */
