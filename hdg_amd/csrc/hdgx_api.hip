@@ -96,7 +96,6 @@ int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) 
   const int nd = m->nd;
   if (a->nd != nd) return fail(ctx, HDGX_ERR_ARG, "app.nd != master.nd");
   if (a->ncu != nd + 2 || a->nch != a->ncu) return fail(ctx, HDGX_ERR_ARG, "app: ncu must be nd+2");
-  if (a->getdqdg != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "app: only getdqdg = 1 (nco = nc) is implemented");
   if (a->physics == HDGX_PHYS_EULER) {
     if (a->nc != a->ncu || a->localsolve != 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "Euler: u-only (localsolve 0, nc = ncu)");
     if (a->nparam < 3) return fail(ctx, HDGX_ERR_ARG, "Euler: param needs {gam, eps, ..., tau}");
@@ -330,7 +329,7 @@ int hdgx_schur_u(hdgx_ctx* ctx, int npv, int ncu, int nc, int ndf, int64_t ne, c
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int nd = ncu - 2;
   const int p = porder_of(nd, npv);
-  SchurArgs sa{ne, nullptr, nullptr, nullptr, BD, F, GK, H, nullptr, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec};
+  SchurArgs sa{ne, nullptr, nullptr, nullptr, BD, F, GK, H, nullptr, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec, nc};
   int rc = with_cfg(0, nd, p, [&](auto cfg) {
     using Cc = decltype(cfg);
     if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: ndf does not match a simplex of this order");
@@ -351,7 +350,7 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
   if (nd != 2 || ncu != 4) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: 2D with ncu = 4 is built");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const int p = porder_of(nd, npv);
-  SchurArgs sa{ne, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec};
+  SchurArgs sa{ne, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg, dudg_duh, ae, fe, status, ctx->prec, nc};
   int rc = with_cfg(1, nd, p, [&](auto cfg) {
     using Cc = decltype(cfg);
     if (Cc::NDF != ndf) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: ndf does not match");
@@ -367,7 +366,9 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
   if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
-  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !ae || !fe || !dudg || !dudg_duh))
+  // DUDG / DUDG_DUH keep nco of the nc rows (hdg_elemental.m:20-26, 146-147)
+  const int nco = app->getdqdg == 1 ? app->nc : (app->getdqdg == 0 ? app->ncu : 0);
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !ae || !fe || (nco > 0 && (!dudg || !dudg_duh))))
     return fail(ctx, HDGX_ERR_ARG, "null array");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   MasterBuf* mb = nullptr;
@@ -377,6 +378,7 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   if ((rc = make_app(ctx, master, app, &ap))) return rc;
   if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
   const int nd = master->nd, npv = master->npv, ncu = app->ncu, nc = app->nc;
+  const size_t ldo = (size_t)npv * nco;
   const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
   const int p = porder_of(nd, npv);
   const bool uq = app->localsolve != 0;
@@ -405,8 +407,9 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
     QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe, udg + (size_t)e0 * npv * nc,
                 uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr, BD, F, GK, H, Ru, Rh, M, C, E, Rq,
                 nullptr};
-    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg + (size_t)e0 * ncol, dudg_duh + (size_t)e0 * ncol * nh,
-                 ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh, status ? status + e0 : nullptr, ctx->prec};
+    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
+                 nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
+                 status ? status + e0 : nullptr, ctx->prec, nco};
     rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
       using Cc = decltype(cfg);
       int r = launch_quad<Cc>(ctx, *mb, ap, qa);
@@ -440,6 +443,7 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
         a->localsolve != a0->localsolve)
       return fail(ctx, HDGX_ERR_ARG, "batch: segments must share physics, nd, ncu, nc and localsolve");
     if ((rc = make_app(ctx, master, a, &apps[s]))) return rc;
+    if (a->getdqdg != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "batch: getdqdg = 1 (nco = nc) only");
     if (g.ne < 0) return fail(ctx, HDGX_ERR_ARG, "batch: ne < 0");
     if (g.ne > 0 && (!g.dgnodes || !g.bf || !g.udg || !g.uh || !g.ae || !g.fe || !g.dudg || !g.dudg_duh))
       return fail(ctx, HDGX_ERR_ARG, "batch: null array");
@@ -582,6 +586,7 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
     pof[s] = p;
     order[s] = s;
     if ((rc = make_app(ctx, masters[s], a, &apps[s]))) return rc;
+    if (a->getdqdg != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: getdqdg = 1 (nco = nc) only");
     if (g.ne < 0) return fail(ctx, HDGX_ERR_ARG, "plan: ne < 0");
     if (g.ne > 0 && (!g.dgnodes || !g.bf || !g.udg || !g.uh || !g.ae || !g.fe || !g.dudg || !g.dudg_duh))
       return fail(ctx, HDGX_ERR_ARG, "plan: null array");

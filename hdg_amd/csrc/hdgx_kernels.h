@@ -85,6 +85,7 @@ struct SchurArgs {
   double *dudg, *dudg_duh, *ae, *fe;
   int32_t* status;
   int32_t prec;                    // 0 fp64, 1 fp32 mfma + fp64 refinement
+  int32_t nco;                     // DUDG / DUDG_DUH rows kept per node: nc, ncu or 0
 };
 
 // One request's element range inside a batched launch: its arrays and the

@@ -331,7 +331,8 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
   const double* Fe = A.F + e * (int64_t)NU * NH;
   const double* GKe = A.GK + e * (int64_t)NH * NCOL;
   const double* He = A.H + e * (int64_t)NH * NH;
-  double* Xe = A.dudg_duh + e * (int64_t)NCOL * NH;
+  const int LDO = NPV * A.nco;   // DUDG / DUDG_DUH rows written (the first LDO of NCOL)
+  double* Xe = A.dudg_duh + e * (int64_t)LDO * NH;
 
   // ---- node sets of the AE row tiles: the volume nodes of the faces met by
   // rows 16*rta .. 16*rta+15 (GK is face sparse); without perm (GK given by a
@@ -785,11 +786,11 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
       for (int cc = 0; cc < 16; ++cc) {
         const int q = q0 + cc;
         if (q >= NRHS) break;
-        for (int r = lane; r < NCOL; r += 64) {
+        for (int r = lane; r < LDO; r += 64) {
           const double v = sXt[r + NCOLP * cc];
           if (!isfinite(v)) st |= 4;
-          if (q == 0) A.dudg[e * NCOL + r] = v;
-          else Xe[r + (int64_t)NCOL * rhs_t<C>(q)] = v;
+          if (q == 0) A.dudg[e * LDO + r] = v;
+          else Xe[r + (int64_t)LDO * rhs_t<C>(q)] = v;
         }
       }
       // AE(:, tile) = H + GK X, FE = Rh - GK x (schur_uq.c:279-285); all AE row

@@ -98,6 +98,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
   A.fe = sg.fe + e * NH;
   A.status = sg.status ? sg.status + e : nullptr;
   A.prec = 0;
+  A.nco = C::NC;
   __shared__ SchurLds<C> lds;
   schur_mfma_body<C>(A, perm, 0, lds);
 }

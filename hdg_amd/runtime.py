@@ -190,14 +190,15 @@ class Context:
         dg, b, u, h = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh)
         s = _dev(sh)
         ne = dg.shape[2]
-        npv, ncu, nc = master.npv, ca.ncu, ca.nc
+        npv, ncu = master.npv, ca.ncu
+        nco = ca.nc if ca.getdqdg == 1 else (ncu if ca.getdqdg == 0 else 0)   # hdg_elemental.m:20-26
         ndf = master.npf * master.nfe
         if out is None:
-            out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
-                   Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+            out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nco, ne)),
+                   Dev.empty((npv, nco, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
         ae, fe, dudg, dudg_duh, st = out
         self._check(lib().hdgx_elemental(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
-                                         h.ptr, _p(s), ae.ptr, fe.ptr, dudg.ptr, dudg_duh.ptr, st.ptr))
+                                         h.ptr, _p(s), ae.ptr, fe.ptr, _p(dudg), _p(dudg_duh), st.ptr))
         return ae, fe, dudg, dudg_duh, st
 
     def _segments(self, segments):

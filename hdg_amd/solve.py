@@ -33,7 +33,13 @@ class HdgState:
         ne = mesh.ne
         ndf = master.npf * master.nfe
         self.npv, self.nc, self.ncu, self.ne, self.ndf = npv, nc, ncu, ne, ndf
-        self.nco = nc                                   # getdqdg = 1 (hdg_elemental.m:20-26)
+        # rows of DUDG kept by the elemental step (hdg_solve.m:56-60); with
+        # getdqdg = 0 q is recovered from (u, uh) by getq after every step
+        if app.getdqdg not in (0, 1):
+            raise HdgxError("getdqdg must be 0 or 1 (hdg_local's element solves are not built)")
+        self.getdqdg = app.getdqdg
+        self.nco = ncu if app.getdqdg == 0 else nc
+        nco = self.nco
         self.ts = TraceSystem(ctx, mesh, ncu, with_bilu0=with_bilu0)
         self.dgnodes = Dev.from_numpy(mesh.dgnodes)
         self.bf = Dev.from_numpy(mesh.bf, np.int32)
@@ -45,8 +51,8 @@ class HdgState:
         self.uhe = Dev.empty((ncu, ndf, ne))
         self.AE = Dev.empty((ncu, ndf, ncu, ndf, ne))
         self.FE = Dev.empty((ncu, ndf, ne))
-        self.DUDG = Dev.empty((npv, nc, ne))
-        self.DUDG_DUH = Dev.empty((npv, nc, ncu, ndf, ne))
+        self.DUDG = Dev.empty((npv, nco, ne))
+        self.DUDG_DUH = Dev.empty((npv, nco, ncu, ndf, ne))
         self.est = Dev.empty((ne,), "int32")
         ts = self.ts
         self.K = Dev.empty((ts.ncf, ts.ncf, ts.nbf - 1, ts.nf))
@@ -54,7 +60,7 @@ class HdgState:
         self.fst = Dev.empty((ts.nf,), "int32")
         self.LU = Dev.empty((ts.ncf, ts.ncf, ts.nbf, ts.nf)) if with_bilu0 else None
         self.DUH = Dev.zeros((ts.ncf, ts.nf))
-        self.DUDGT = Dev.empty((npv, nc, ne))
+        self.DUDGT = Dev.empty((npv, nco, ne))
         self.UDG0 = Dev.empty((npv, nc, ne))
         self.UH0 = Dev.empty((ts.ncf, ts.nf))
 
@@ -92,6 +98,26 @@ class HdgState:
         s0, h0 = (self.UDG0, self.UH0) if base else (self.UDG, self.UH)
         self.ts.newton_update(self.npv, self.nc, self.nco, alpha, s0, self.DUDGT, h0, self.DUH, self.UDG, self.UH)
 
+    def getq(self):
+        """UDG(:, ncu+1:end, :) = getq(master, mesh, UDG, UH, [], app.fc_q)
+        (hdg_solve.m:130-131, 149-150; hdg_solve_dirk.m:89-90).  No-op without
+        q columns (nc = ncu)."""
+        if self.nc == self.ncu:
+            return
+        self.ts.gather(self.UH, self.uhe)
+        self.ctx.getq(self.master, self.dgnodes, self.UDG, self.uhe, fc_q=self.app.fc_q, ncu=self.ncu, inplace=True)
+
+    def set_source(self, SH):
+        """SH for the next solve, q columns zeroed (hdg_solve.m:63-64, app.wave = 0).
+        SH: a torch tensor or Dev of shape (npv, nc, ne)."""
+        t = SH.t if isinstance(SH, Dev) else SH
+        self.SH.t.copy_(t.reshape(-1))
+        self.SH.t.view(self.ne, self.nc, self.npv)[:, self.ncu:, :] = 0.0
+
+    def solve(self, **newton):
+        """The damped Newton solve of hdg_solve.m on this state (hdg_solve below)."""
+        return hdg_solve(self, **newton)
+
     def save(self):
         self.UDG0.t.copy_(self.UDG.t)
         self.UH0.t.copy_(self.UH.t)
@@ -122,6 +148,8 @@ def hdg_solve(state, newton_tol=1e-9, maxiter=16, restart=20, gmres_tol=1e-6, pr
         alpha = 1.0
         while True:
             state.step(alpha)
+            if state.getdqdg == 0:
+                state.getq()
             state.assemble()
             duh = state.residual()
             if duh > duh0:

@@ -71,7 +71,8 @@ typedef struct hdgx_app {
   int32_t localsolve; /* 0: u only (schur_u), 1: u and q (schur_uq) */
   int32_t tdep;       /* 1: add the BDF source s += SH - fc_u*u (volint.m:180-221) */
   int32_t av_col;     /* dgnodes column with the artificial viscosity, or -1 */
-  int32_t getdqdg;    /* 1: nco = nc (hdg_elemental.m:20-26); only 1 is supported */
+  int32_t getdqdg;    /* DUDG rows kept (hdg_elemental.m:20-26): 1 nco = nc, 0 nco = ncu (q
+                         recovered by hdgx_getq), other nco = 0; batched launches: 1 only */
   double fc_u, fc_q, time;
   int32_t nparam;
   double param[HDGX_MAX_PARAM]; /* Euler {gam, eps, tau}; NS {gam, eps, Re, Pr, Minf, tau} */
@@ -104,7 +105,8 @@ const char* hdgx_last_error(hdgx_ctx* ctx);
  *   udg (npv, nc, ne), uh (ncu, npf*nfe, ne) gathered trace (hdg_assemble.m:34),
  *   sh (npv, nc, ne) (read only when app->tdep or localsolve);
  *   ae (ncu, ndf, ncu, ndf, ne), fe (ncu, ndf, ne),
- *   dudg (npv, nc, ne), dudg_duh (npv, nc, ncu, ndf, ne), status (ne) int32. */
+ *   dudg (npv, nco, ne), dudg_duh (npv, nco, ncu, ndf, ne) with nco = nc, ncu or 0
+ *   as app->getdqdg is 1, 0 or other (may be NULL when nco = 0), status (ne) int32. */
 int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
                    const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
                    const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
