@@ -156,6 +156,32 @@ def residual_leg(torch, ctx, P, dev, steps, warmup):
             "max_rel_diff_vs_element_matrices": err}
 
 
+def getq_leg(torch, ctx, P, dev, steps, warmup):
+    """q recovery (hdgx_getq, kernel/getq.m) on the same elements: time, its
+    HBM rate against the algorithmic bytes (node coordinates, u, gathered
+    trace and SH_q in; q out), and the local residual Rq with the recovered q
+    put into UDG, relative to Rq with q = 0 (zero up to rounding)."""
+    from hdg_amd.runtime import Dev
+    npv, ncu, nd, ne, ndf = P.master.npv, P.app.ncu, P.master.nd, P.ne, P.master.ndf
+    qout = Dev.empty((npv, ncu * nd, ne))
+    ms, _ = time_steps(torch, ctx.stream, lambda: ctx.getq(P.master, dev[0], dev[2], dev[3], sh=dev[4],
+                                                           fc_q=P.app.fc_q, out=qout), steps, warmup)
+    nbytes = 8 * (npv * nd + npv * ncu + ncu * ndf + 2 * npv * ncu * nd) * ne    # + SH_q in
+    udg = Dev(dev[2].t.clone(), dev[2].shape)
+    ctx.getq(P.master, dev[0], udg, dev[3], sh=dev[4], fc_q=P.app.fc_q, inplace=True)
+    rq1 = ctx.residual(P.master, P.app, dev[0], dev[1], udg, dev[3], dev[4])[2]
+    udg.t.view(ne, -1, npv)[:, ncu:, :] = 0.0
+    rqz = ctx.residual(P.master, P.app, dev[0], dev[1], udg, dev[3], dev[4])[2]
+    torch.cuda.synchronize()
+    red = float(rq1.t.abs().max() / rqz.t.abs().max())
+    del udg, rq1, rqz
+    log(f"[bench] getq: {ms:.3f} ms/step, {ne / ms * 1e3:.4g} elements/s, |Rq(q)| / |Rq(0)| = {red:.2e}")
+    return {"workload": "getq q recovery (q = (fc_q M)^-1 (C u - E uh)), same elements",
+            "ms_per_step": ms, "value": ne / (ms * 1e-3), "unit": "elements/s",
+            "hbm_gbs_algorithmic": nbytes / (ms * 1e-3) / 1e9, "bytes_per_element": nbytes // ne,
+            "rq_over_rq_at_q0": red}
+
+
 def serving_leg(nreq, seed):
     """Poisson stream of BDF1 Navier-Stokes Newton steps through the
     continuous-batching loop: p50/p99 latency and throughput."""
@@ -326,6 +352,7 @@ def main():
     tq, ts = kernel_times(torch, ctx, P, dev, out, reps=5)
     log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
     resid = residual_leg(torch, ctx, P, dev, a.steps, a.warmup)
+    getq = getq_leg(torch, ctx, P, dev, a.steps, a.warmup)
 
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ref, cpu_rate, cpu_rate1, build_kind, cpu_secs, noise_floor = cpu_baseline(P, ncpu)
@@ -377,6 +404,7 @@ def main():
                          "single_core_value": cpu_rate1, "single_core_sample": f"first {min(2048, ne)} elements"},
         "speedup_vs_cpu": value / cpu_rate,
         "residual": resid,
+        "getq": getq,
     }
     del out, dev
     torch.cuda.empty_cache()

@@ -30,6 +30,11 @@
 
 namespace hdgx {
 
+// One wavefront per element: the per-element work is a chain of small
+// barrier-separated steps, so many resident elements (not wide ones) hide the
+// latency.
+constexpr int GETQ_THREADS = 64;
+
 template <class C>
 struct GetqLds {
   static constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
@@ -51,11 +56,11 @@ struct GetqLds {
 };
 
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) void getq_kernel(GetqArgs A, DevMaster T) {
+__global__ __launch_bounds__(GETQ_THREADS) void getq_kernel(GetqArgs A, DevMaster T) {
   using L = GetqLds<C>;
   constexpr int ND = L::ND, NPV = L::NPV, NPF = L::NPF, NFE = L::NFE, NDF = L::NDF;
   constexpr int NGV = L::NGV, NGF = L::NGF, NCU = L::NCU, NFG = L::NFG, NQ = L::NQ, NA = L::NA;
-  constexpr int NH = NCU * NDF, NT = QUAD_THREADS;
+  constexpr int NH = NCU * NDF, NT = GETQ_THREADS;
   __shared__ L lds;
   double* const sX = lds.sX;
   double* const sU = lds.sU;
@@ -76,24 +81,13 @@ __global__ __launch_bounds__(QUAD_THREADS) void getq_kernel(GetqArgs A, DevMaste
   const int tid = threadIdx.x;
   const int nc = A.nc, ncd = A.ncd;
 
-  // ---- 0. stage (every array fits one pass: loads issued before the stores)
-  {
-    static_assert(NPV * ND <= NT && NPV * NCU <= NT && NH <= NT && NGV * NPV <= NT && NGF * NPF <= NT &&
-                  NPV * NFE <= NT, "staged arrays fit one workgroup pass");
-    const int xi = tid % NPV, xd = tid / NPV;
-    const double x = tid < NPV * ND ? A.dgnodes[e * NPV * ncd + xi + NPV * xd] : 0.0;
-    const double u = tid < NPV * NCU ? A.udg[e * NPV * nc + tid] : 0.0;
-    const double uh = tid < NH ? A.uh[e * NH + tid] : 0.0;
-    const double shv = tid < NGV * NPV ? T.shapvt[tid] : 0.0;
-    const double shf = tid < NGF * NPF ? T.shapft[tid] : 0.0;
-    const int32_t loc = tid < NPV * NFE ? T.loc[tid] : 0;
-    if (tid < NPV * ND) sX[tid] = x;
-    if (tid < NPV * NCU) sU[tid] = u;
-    if (tid < NH) sUH[tid] = uh;
-    if (tid < NGV * NPV) sShV[tid] = shv;
-    if (tid < NGF * NPF) sShF[tid] = shf;
-    if (tid < NPV * NFE) sLoc[tid] = loc;
-  }
+  // ---- 0. stage
+  for (int i = tid; i < NPV * ND; i += NT) sX[i] = A.dgnodes[e * NPV * ncd + i];
+  for (int i = tid; i < NPV * NCU; i += NT) sU[i] = A.udg[e * NPV * nc + i];
+  for (int i = tid; i < NH; i += NT) sUH[i] = A.uh[e * NH + i];
+  for (int i = tid; i < NGV * NPV; i += NT) sShV[i] = T.shapvt[i];
+  for (int i = tid; i < NGF * NPF; i += NT) sShF[i] = T.shapft[i];
+  for (int i = tid; i < NPV * NFE; i += NT) sLoc[i] = T.loc[i];
   __syncthreads();
 
   // ---- 1. geometry and the state at the quadrature points

@@ -24,7 +24,7 @@ hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, co
 
 template <class C>
 hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
-  hipLaunchKernelGGL(getq_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t);
+  hipLaunchKernelGGL(getq_kernel<C>, dim3((unsigned)a.ne), dim3(GETQ_THREADS), 0, s, a, t);
   return hipGetLastError();
 }
 
