@@ -246,6 +246,13 @@ int hdgx_set_precision(hdgx_ctx* ctx, int precision) {
   return HDGX_OK;
 }
 
+int hdgx_set_flags(hdgx_ctx* ctx, int flags) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (flags & ~HDGX_INPLACE) return fail(ctx, HDGX_ERR_ARG, "flags");
+  ctx->flags = flags;
+  return HDGX_OK;
+}
+
 const char* hdgx_last_error(hdgx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
 int hdgx_elemental_matrices(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
@@ -336,6 +343,12 @@ int hdgx_schur_u(hdgx_ctx* ctx, int npv, int ncu, int nc, int ndf, int64_t ne, c
     return launch_schur<Cc>(ctx, sa, nullptr);
   });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_u: no kernel for these sizes");
+  if (rc == HDGX_OK && (ctx->flags & HDGX_INPLACE) && ne > 0) {
+    // schur_u.c:67: BD -> its LU (the caller's buffer, declared const, is written)
+    InplaceArgs ia{ne, npv, ncu, nd, ndf, 0, nullptr, nullptr, nullptr, const_cast<double*>(BD), nullptr, nullptr,
+                   nullptr};
+    HIP_TRY(ctx, inplace_launch(ctx->stream, ia));
+  }
   return rc;
 }
 
@@ -357,6 +370,13 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
     return launch_schur<Cc>(ctx, sa, nullptr);
   });
   if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: no kernel for these sizes");
+  if (rc == HDGX_OK && (ctx->flags & HDGX_INPLACE) && ne > 0) {
+    // schur_uq.c:localsolveuq's side effects on M, C, E, BD, F, Rq, Ru (the
+    // caller's buffers, declared const, are written)
+    InplaceArgs ia{ne, npv, ncu, nd, ndf, 1, const_cast<double*>(M), const_cast<double*>(C), const_cast<double*>(E),
+                   const_cast<double*>(BD), const_cast<double*>(F), const_cast<double*>(Rq), const_cast<double*>(Ru)};
+    HIP_TRY(ctx, inplace_launch(ctx->stream, ia));
+  }
   return rc;
 }
 

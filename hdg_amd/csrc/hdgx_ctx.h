@@ -21,6 +21,7 @@ struct hdgx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int prec = HDGX_PREC_FP64;
+  int flags = 0;                 // HDGX_INPLACE
   std::string err;
   std::map<uint64_t, MasterBuf> masters;
   void* ws = nullptr;            // element matrices between the quadrature and Schur kernels

@@ -79,6 +79,14 @@ struct GetqArgs {
   double* qq;                      // (npv, ncu*nd, ne) or null
 };
 
+// HDGX_INPLACE: the MEX files' side effects on their inputs (schur_uq.c,
+// schur_u.c); uq = 0 for schur_u (only BD is written).
+struct InplaceArgs {
+  int64_t ne;
+  int32_t npv, ncu, nd, ndf, uq;
+  double *M, *C, *E, *BD, *F, *Rq, *Ru;
+};
+
 struct SchurArgs {
   int64_t ne;
   const double *M, *C, *E, *BD, *F, *GK, *H, *Rq, *Ru, *Rh;

@@ -21,6 +21,8 @@ namespace hdgx {
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
 // residual only (Ru, Rh with hdg_residual's sign, Rq)
 template <class C> hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
+// the MEX files' in-place side effects (one launch after the elimination)
+hipError_t inplace_launch(hipStream_t s, const InplaceArgs& a);
 // q recovery; geometry only, so it is built for the Euler configurations
 // (PH = 0), whose (nd, p) cover every built order
 template <class C> hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t);

@@ -22,6 +22,7 @@ LIB = os.path.join(HERE, "libhdgx.so")
 
 PREC_FP64 = 0
 PREC_FP32_IR = 1
+INPLACE = 1          # HDGX_INPLACE
 
 _lib = None
 
@@ -41,6 +42,7 @@ def lib():
         L.hdgx_destroy.argtypes = [vp]
         L.hdgx_set_stream.argtypes = [vp, vp]
         L.hdgx_set_precision.argtypes = [vp, i32]
+        L.hdgx_set_flags.argtypes = [vp, i32]
         L.hdgx_last_error.argtypes = [vp]
         L.hdgx_last_error.restype = ctypes.c_char_p
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
@@ -68,7 +70,7 @@ def lib():
     return _lib
 
 
-EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_last_error",
+EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
            "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
@@ -159,6 +161,11 @@ class Context:
     def set_precision(self, precision):
         self.precision = precision
         self._check(lib().hdgx_set_precision(self.h, precision))
+
+    def set_flags(self, flags):
+        """hdgx_set_flags: INPLACE makes schur_u / schur_uq overwrite their
+        (device) inputs as the MEX files do."""
+        self._check(lib().hdgx_set_flags(self.h, int(flags)))
 
     def _check(self, rc):
         if rc:

@@ -363,8 +363,10 @@ class Server:
                         used -= slot.bytes
                         finished.append(slot)
                         if log:
+                            t_arr = slot.spec.t_arrival if realtime else 0.0
                             log(f"[serve] req {slot.spec.rid} p={slot.spec.p} ne={slot.spec.ne} done "
-                                f"{(slot.t_done - (slot.spec.t_arrival if realtime else 0)) * 1e3:.1f} ms")
+                                f"{(slot.t_done - t_arr) * 1e3:.1f} ms (waited {(slot.t_admit - t_arr) * 1e3:.1f} ms, "
+                                f"gmres {slot.gmres_iters})")
                     try:
                         item = doneq.get_nowait()
                     except queue.Empty:

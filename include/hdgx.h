@@ -34,6 +34,9 @@ enum { HDGX_PHYS_EULER = 0, HDGX_PHYS_NS = 1 };
 /* arithmetic of the local elimination (see hdgx_set_precision) */
 enum { HDGX_PREC_FP64 = 0, HDGX_PREC_FP32_IR = 1 };
 
+/* context flags (see hdgx_set_flags) */
+enum { HDGX_INPLACE = 1 };
+
 /* per-element status bits written to status[e] (0 = clean).  The reference
  * ignores LAPACK's info (kernel/schur_uq.c:69,136); these report it instead. */
 enum {
@@ -96,6 +99,18 @@ int hdgx_set_stream(hdgx_ctx* ctx, void* stream);
  *                      there); AE, FE and the q rows stay FP64.
  * The quadrature is FP64 in both. */
 int hdgx_set_precision(hdgx_ctx* ctx, int precision);
+/* Flags of later hdgx_schur_u / hdgx_schur_uq calls on ctx (0 = none):
+ *   HDGX_INPLACE  also overwrite the inputs the way the MEX files do, for
+ *                 comparisons with them (kernel/schur_uq.c:69-136,
+ *                 kernel/schur_u.c:67): M -> its dpotrf('U') factor (upper
+ *                 triangle; the strict lower one untouched), C, E, Rq ->
+ *                 M^-1 C, M^-1 E, M^-1 Rq, Ru -> Ru - B M^-1 Rq, F -> B M^-1 E
+ *                 - F, and the D block of BD -> dgetrf's LU of D + B M^-1 C
+ *                 (schur_u: of D = BD).  The pointers are declared const
+ *                 but are written.  One more launch per call; the outputs
+ *                 are unchanged.  hdgx_elemental is unaffected (its Schur
+ *                 inputs are internal). */
+int hdgx_set_flags(hdgx_ctx* ctx, int flags);
 const char* hdgx_last_error(hdgx_ctx* ctx);
 
 /* Elemental operator.  Replaces
