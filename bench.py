@@ -182,6 +182,38 @@ def getq_leg(torch, ctx, P, dev, steps, warmup):
             "rq_over_rq_at_q0": red}
 
 
+def host_leg(torch, ctx, P, reps=2):
+    """hdgx_elemental on host (pageable numpy) arrays, HDGX_HOST_PTRS: the
+    rate a caller handing over host buffers sees, transfers included."""
+    import ctypes
+    from hdg_amd.app import MasterC
+    from hdg_amd.runtime import lib
+    m, a = P.master, P.app
+    npv, nc, ne, nh = m.npv, a.nc, P.ne, a.ncu * m.ndf
+    z = lambda *sh: np.zeros(sh, order="F")
+    out = [z(nh, nh, ne), z(nh, ne), z(npv, nc, ne), z(npv, nc, nh, ne), np.zeros(ne, dtype=np.int32)]
+    ins = [np.asfortranarray(x) for x in (P.mesh.dgnodes, P.udg, P.uh, P.sh)]
+    bf = np.asfortranarray(P.mesh.bf.astype(np.int32))
+    mc, ca = MasterC(m), a.to_c()
+    p = lambda x: x.ctypes.data_as(ctypes.c_void_p)
+    call = lambda: lib().hdgx_elemental(ctx.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, p(ins[0]), p(bf), p(ins[1]),
+                                        p(ins[2]), p(ins[3]), *[p(o) for o in out])
+    ctx.set_flags(2)                      # HDGX_HOST_PTRS
+    try:
+        ctx._check(call())
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx._check(call())
+        dt = (time.perf_counter() - t0) / reps
+    finally:
+        ctx.set_flags(0)
+    nbytes = sum(x.nbytes for x in ins) + bf.nbytes + sum(o.nbytes for o in out)
+    log(f"[bench] host buffers: {dt * 1e3:.1f} ms/step, {ne / dt:.4g} elements/s, {nbytes / dt / 1e9:.1f} GB/s over PCIe")
+    return {"workload": "hdgx_elemental on pageable host arrays (HDGX_HOST_PTRS), same elements",
+            "ms_per_step": dt * 1e3, "value": ne / dt, "unit": "elements/s", "bytes_per_step": nbytes,
+            "host_link_gbs": nbytes / dt / 1e9}
+
+
 def serving_leg(nreq, seed):
     """Poisson stream of BDF1 Navier-Stokes Newton steps through the
     continuous-batching loop: p50/p99 latency and throughput."""
@@ -353,6 +385,7 @@ def main():
     log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
     resid = residual_leg(torch, ctx, P, dev, a.steps, a.warmup)
     getq = getq_leg(torch, ctx, P, dev, a.steps, a.warmup)
+    host = host_leg(torch, ctx, P)
 
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ref, cpu_rate, cpu_rate1, build_kind, cpu_secs, noise_floor = cpu_baseline(P, ncpu)
@@ -405,6 +438,7 @@ def main():
         "speedup_vs_cpu": value / cpu_rate,
         "residual": resid,
         "getq": getq,
+        "host_buffers": host,
     }
     del out, dev
     torch.cuda.empty_cache()

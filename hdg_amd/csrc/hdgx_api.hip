@@ -9,6 +9,7 @@
 #include <mutex>
 #include <algorithm>
 #include <cmath>
+#include <functional>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -207,6 +208,73 @@ int ensure_ws(hdgx_ctx* ctx, size_t bytes) {
   return HDGX_OK;
 }
 
+// HDGX_HOST_PTRS: an entry point's array arguments are host memory.  The
+// arrays are given device copies in the context's staging buffer (inputs
+// copied in, outputs copied back after the call, which then synchronises),
+// and the call runs on those copies with the flag cleared.
+class HostStage {
+ public:
+  explicit HostStage(hdgx_ctx* ctx) : ctx_(ctx) {}
+  template <class T> void in(const T*& p, size_t n) { add(p, n * sizeof(T), true, false, [&p](void* d) { p = (const T*)d; }); }
+  template <class T> void out(T*& p, size_t n) { add(p, n * sizeof(T), false, true, [&p](void* d) { p = (T*)d; }); }
+  template <class T> void inout(T*& p, size_t n) { add(p, n * sizeof(T), true, true, [&p](void* d) { p = (T*)d; }); }
+
+  // copies the inputs in and points every registered argument at its copy
+  int bind() {
+    hdgx_ctx* ctx = ctx_;
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    if (total_ > ctx->hws_bytes) {
+      if (ctx->hws) HIP_TRY(ctx, hipFree(ctx->hws));
+      ctx->hws = nullptr;
+      ctx->hws_bytes = 0;
+      HIP_TRY(ctx, hipMalloc(&ctx->hws, total_));
+      ctx->hws_bytes = total_;
+    }
+    char* base = static_cast<char*>(ctx->hws);
+    for (Item& it : items_) {
+      it.dev = base + it.off;
+      if (it.in) HIP_TRY(ctx, hipMemcpyAsync(it.dev, it.host, it.bytes, hipMemcpyHostToDevice, ctx->stream));
+      it.set(it.dev);
+    }
+    return HDGX_OK;
+  }
+  // copies the outputs back and waits for them
+  int finish() {
+    hdgx_ctx* ctx = ctx_;
+    for (Item& it : items_)
+      if (it.out) HIP_TRY(ctx, hipMemcpyAsync(const_cast<void*>(it.host), it.dev, it.bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return HDGX_OK;
+  }
+
+ private:
+  struct Item {
+    const void* host;
+    size_t bytes, off;
+    bool in, out;
+    std::function<void(void*)> set;
+    void* dev;
+  };
+  void add(const void* host, size_t bytes, bool in, bool out, std::function<void(void*)> set) {
+    if (!host || bytes == 0) return;
+    items_.push_back({host, bytes, total_, in, out, std::move(set), nullptr});
+    total_ += (bytes + 255) / 256 * 256;
+  }
+  hdgx_ctx* ctx_;
+  std::vector<Item> items_;
+  size_t total_ = 0;
+};
+
+// runs `call` with HDGX_HOST_PTRS cleared, then restores the flags
+template <class F>
+int without_host_ptrs(hdgx_ctx* ctx, F&& call) {
+  const int saved = ctx->flags;
+  ctx->flags &= ~HDGX_HOST_PTRS;
+  const int rc = call();
+  ctx->flags = saved;
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -228,6 +296,7 @@ int hdgx_destroy(hdgx_ctx* ctx) {
   if (ctx->ws) (void)hipFree(ctx->ws);
   if (ctx->tws) (void)hipFree(ctx->tws);
   if (ctx->segdev) (void)hipFree(ctx->segdev);
+  if (ctx->hws) (void)hipFree(ctx->hws);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
   return HDGX_OK;
@@ -248,7 +317,7 @@ int hdgx_set_precision(hdgx_ctx* ctx, int precision) {
 
 int hdgx_set_flags(hdgx_ctx* ctx, int flags) {
   if (!ctx) return HDGX_ERR_ARG;
-  if (flags & ~HDGX_INPLACE) return fail(ctx, HDGX_ERR_ARG, "flags");
+  if (flags & ~(HDGX_INPLACE | HDGX_HOST_PTRS)) return fail(ctx, HDGX_ERR_ARG, "flags");
   ctx->flags = flags;
   return HDGX_OK;
 }
@@ -261,6 +330,22 @@ int hdgx_elemental_matrices(hdgx_ctx* ctx, const hdgx_master* master, const hdgx
                             double* Rh, double* M, double* C, double* E, double* Rq) {
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (ctx->flags & HDGX_HOST_PTRS) {
+    if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+    const size_t n = (size_t)ne, npv = master->npv, ndf = (size_t)master->npf * master->nfe, nd = master->nd;
+    const size_t ncu = app->ncu, nc = app->nc, nh = ncu * ndf;
+    HostStage st(ctx);
+    st.in(dgnodes, npv * app->ncd * n); st.in(bf, master->nfe * n); st.in(udg, npv * nc * n); st.in(uh, nh * n);
+    st.in(sh, npv * nc * n);
+    st.out(BD, npv * ncu * npv * nc * n); st.out(F, npv * ncu * nh * n); st.out(GK, nh * npv * nc * n);
+    st.out(H, nh * nh * n); st.out(Ru, npv * ncu * n); st.out(Rh, nh * n); st.out(M, npv * npv * n);
+    st.out(C, npv * npv * nd * n); st.out(E, npv * ndf * nd * n); st.out(Rq, npv * ncu * nd * n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return hdgx_elemental_matrices(ctx, master, app, ne, dgnodes, bf, udg, uh, sh, BD, F, GK, H, Ru, Rh, M, C, E, Rq);
+    });
+    return rc ? rc : st.finish();
+  }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   MasterBuf* mb = nullptr;
   int rc = get_master(ctx, master, &mb);
@@ -283,6 +368,20 @@ int hdgx_residual(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app,
                   double* Rq) {
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (ctx->flags & HDGX_HOST_PTRS) {
+    if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+    const size_t n = (size_t)ne, npv = master->npv, ndf = (size_t)master->npf * master->nfe, nd = master->nd;
+    const size_t ncu = app->ncu, nc = app->nc, nh = ncu * ndf;
+    HostStage st(ctx);
+    st.in(dgnodes, npv * app->ncd * n); st.in(bf, master->nfe * n); st.in(udg, npv * nc * n); st.in(uh, nh * n);
+    st.in(sh, npv * nc * n);
+    st.out(Ru, npv * ncu * n); st.out(Rh, nh * n); st.out(Rq, npv * ncu * nd * n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return hdgx_residual(ctx, master, app, ne, dgnodes, bf, udg, uh, sh, Ru, Rh, Rq);
+    });
+    return rc ? rc : st.finish();
+  }
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   MasterBuf* mb = nullptr;
   int rc = get_master(ctx, master, &mb);
@@ -307,6 +406,21 @@ int hdgx_getq(hdgx_ctx* ctx, const hdgx_master* master, int ncu, int nc, int ncd
   if (!master) return fail(ctx, HDGX_ERR_ARG, "master: null");
   const int nd = master->nd, ncq = ncu * nd;
   if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if ((ctx->flags & HDGX_HOST_PTRS) && ne > 0 && ldq >= (int64_t)master->npv * ncq && ncu > 0 && nc > 0) {
+    const size_t n = (size_t)ne, npv = master->npv, ndf = (size_t)master->npf * master->nfe;
+    HostStage st(ctx);
+    st.in(dgnodes, npv * ncd * n); st.in(udg, npv * nc * n); st.in(uh, ncu * ndf * n); st.in(sh, npv * nc * n);
+    st.in(shapnv, npv * npv * (nd + 1));
+    // the q blocks may sit inside udg (ldq = npv*nc): the whole strided range
+    // goes in and out so that the columns between them are kept
+    st.inout(qdg, (n - 1) * ldq + npv * ncq);
+    st.out(qq, npv * ncq * n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return hdgx_getq(ctx, master, ncu, nc, ncd, ne, dgnodes, udg, uh, sh, fc_q, qdg, ldq, shapnv, qq);
+    });
+    return rc ? rc : st.finish();
+  }
   if (ncu != nd + 2) return fail(ctx, HDGX_ERR_UNSUPPORTED, "getq: ncu must be nd+2");
   if (nc < ncu) return fail(ctx, HDGX_ERR_ARG, "getq: nc < ncu");
   if (sh && nc < ncq) return fail(ctx, HDGX_ERR_ARG, "getq: sh needs nc >= ncu*nd");
@@ -331,6 +445,22 @@ int hdgx_schur_u(hdgx_ctx* ctx, int npv, int ncu, int nc, int ndf, int64_t ne, c
                  double* dudg_duh, double* ae, double* fe, int32_t* status) {
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0 || nc != ncu) return fail(ctx, HDGX_ERR_ARG, "schur_u: nc must equal ncu");
+  if (ctx->flags & HDGX_HOST_PTRS) {
+    const size_t n = (size_t)ne, nu = (size_t)npv * ncu, nh = (size_t)ncu * ndf, ncol = (size_t)npv * nc;
+    const bool inplace = ctx->flags & HDGX_INPLACE;
+    HostStage st(ctx);
+    double* bd = const_cast<double*>(BD);
+    if (inplace) st.inout(bd, nu * ncol * n); else st.in(BD, nu * ncol * n);
+    st.in(F, nu * nh * n); st.in(GK, nh * ncol * n); st.in(H, nh * nh * n); st.in(Ru, nu * n); st.in(Rh, nh * n);
+    st.out(dudg, ncol * n); st.out(dudg_duh, ncol * nh * n); st.out(ae, nh * nh * n); st.out(fe, nh * n);
+    st.out(status, n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return hdgx_schur_u(ctx, npv, ncu, nc, ndf, ne, inplace ? bd : BD, F, GK, H, Ru, Rh, dudg, dudg_duh, ae, fe,
+                          status);
+    });
+    return rc ? rc : st.finish();
+  }
   if (ne > 0 && (!BD || !F || !GK || !H || !Ru || !Rh || !dudg || !dudg_duh || !ae || !fe))
     return fail(ctx, HDGX_ERR_ARG, "null array");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -358,6 +488,34 @@ int hdgx_schur_uq(hdgx_ctx* ctx, int nd, int npv, int ncu, int nc, int ndf, int6
                   double* dudg_duh, double* ae, double* fe, int32_t* status) {
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0 || nc != ncu * (nd + 1)) return fail(ctx, HDGX_ERR_ARG, "schur_uq: nc must be ncu*(nd+1)");
+  if (ctx->flags & HDGX_HOST_PTRS) {
+    const size_t n = (size_t)ne, nu = (size_t)npv * ncu, nh = (size_t)ncu * ndf, ncol = (size_t)npv * nc;
+    const bool inplace = ctx->flags & HDGX_INPLACE;
+    HostStage st(ctx);
+    // the arrays HDGX_INPLACE writes go both ways
+    double *m = const_cast<double*>(M), *c = const_cast<double*>(C), *e = const_cast<double*>(E);
+    double *bd = const_cast<double*>(BD), *f = const_cast<double*>(F), *rq = const_cast<double*>(Rq);
+    double* ru = const_cast<double*>(Ru);
+    const size_t sm = (size_t)npv * npv * n, sc = sm * nd, se = (size_t)npv * ndf * nd * n;
+    const size_t sbd = nu * ncol * n, sf = nu * nh * n, srq = nu * nd * n, sru = nu * n;
+    if (inplace) {
+      st.inout(m, sm); st.inout(c, sc); st.inout(e, se); st.inout(bd, sbd); st.inout(f, sf); st.inout(rq, srq);
+      st.inout(ru, sru);
+    } else {
+      st.in(M, sm); st.in(C, sc); st.in(E, se); st.in(BD, sbd); st.in(F, sf); st.in(Rq, srq); st.in(Ru, sru);
+    }
+    st.in(GK, nh * ncol * n); st.in(H, nh * nh * n); st.in(Rh, nh * n);
+    st.out(dudg, ncol * n); st.out(dudg_duh, ncol * nh * n); st.out(ae, nh * nh * n); st.out(fe, nh * n);
+    st.out(status, n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return inplace ? hdgx_schur_uq(ctx, nd, npv, ncu, nc, ndf, ne, m, c, e, bd, f, GK, H, rq, ru, Rh, dudg, dudg_duh,
+                                     ae, fe, status)
+                     : hdgx_schur_uq(ctx, nd, npv, ncu, nc, ndf, ne, M, C, E, BD, F, GK, H, Rq, Ru, Rh, dudg, dudg_duh,
+                                     ae, fe, status);
+    });
+    return rc ? rc : st.finish();
+  }
   if (ne > 0 && (!M || !C || !E || !BD || !F || !GK || !H || !Rq || !Ru || !Rh || !dudg || !dudg_duh || !ae || !fe))
     return fail(ctx, HDGX_ERR_ARG, "null array");
   if (nd != 2 || ncu != 4) return fail(ctx, HDGX_ERR_UNSUPPORTED, "schur_uq: 2D with ncu = 4 is built");
@@ -386,6 +544,21 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   if (!ctx) return HDGX_ERR_ARG;
   if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
   if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+  if (ctx->flags & HDGX_HOST_PTRS) {
+    const size_t n = (size_t)ne, npv = master->npv, ndf = (size_t)master->npf * master->nfe;
+    const size_t ncu = app->ncu, nc = app->nc, nh = ncu * ndf;
+    const size_t nco = app->getdqdg == 1 ? nc : (app->getdqdg == 0 ? ncu : 0);
+    HostStage st(ctx);
+    st.in(dgnodes, npv * app->ncd * n); st.in(bf, master->nfe * n); st.in(udg, npv * nc * n); st.in(uh, nh * n);
+    st.in(sh, npv * nc * n);
+    st.out(ae, nh * nh * n); st.out(fe, nh * n); st.out(dudg, npv * nco * n); st.out(dudg_duh, npv * nco * nh * n);
+    st.out(status, n);
+    int rc = st.bind();
+    if (!rc) rc = without_host_ptrs(ctx, [&] {
+      return hdgx_elemental(ctx, master, app, ne, dgnodes, bf, udg, uh, sh, ae, fe, dudg, dudg_duh, status);
+    });
+    return rc ? rc : st.finish();
+  }
   // DUDG / DUDG_DUH keep nco of the nc rows (hdg_elemental.m:20-26, 146-147)
   const int nco = app->getdqdg == 1 ? app->nc : (app->getdqdg == 0 ? app->ncu : 0);
   if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !ae || !fe || (nco > 0 && (!dudg || !dudg_duh))))

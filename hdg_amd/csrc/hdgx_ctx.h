@@ -21,7 +21,7 @@ struct hdgx_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   int prec = HDGX_PREC_FP64;
-  int flags = 0;                 // HDGX_INPLACE
+  int flags = 0;                 // HDGX_INPLACE, HDGX_HOST_PTRS
   std::string err;
   std::map<uint64_t, MasterBuf> masters;
   void* ws = nullptr;            // element matrices between the quadrature and Schur kernels
@@ -31,6 +31,8 @@ struct hdgx_ctx {
   double* pinned = nullptr;      // small host staging for reductions
   void* segdev = nullptr;        // segment and physics tables of batched launches
   size_t segdev_bytes = 0;
+  void* hws = nullptr;           // device copies of host arrays (HDGX_HOST_PTRS)
+  size_t hws_bytes = 0;
 };
 
 namespace hdgx_detail {

@@ -35,7 +35,7 @@ enum { HDGX_PHYS_EULER = 0, HDGX_PHYS_NS = 1 };
 enum { HDGX_PREC_FP64 = 0, HDGX_PREC_FP32_IR = 1 };
 
 /* context flags (see hdgx_set_flags) */
-enum { HDGX_INPLACE = 1 };
+enum { HDGX_INPLACE = 1, HDGX_HOST_PTRS = 2 };
 
 /* per-element status bits written to status[e] (0 = clean).  The reference
  * ignores LAPACK's info (kernel/schur_uq.c:69,136); these report it instead. */
@@ -109,7 +109,16 @@ int hdgx_set_precision(hdgx_ctx* ctx, int precision);
  *                 (schur_u: of D = BD).  The pointers are declared const
  *                 but are written.  One more launch per call; the outputs
  *                 are unchanged.  hdgx_elemental is unaffected (its Schur
- *                 inputs are internal). */
+ *                 inputs are internal).
+ *   HDGX_HOST_PTRS  the array arguments of hdgx_elemental,
+ *                 hdgx_elemental_matrices, hdgx_residual, hdgx_getq,
+ *                 hdgx_schur_u and hdgx_schur_uq are host memory: they are
+ *                 copied to a device staging buffer owned by ctx, the call
+ *                 runs there, and the outputs (and the arrays HDGX_INPLACE
+ *                 writes) are copied back before it returns.  Without the
+ *                 flag every array is device memory and calls are
+ *                 asynchronous.  The trace-solve entry points always take
+ *                 device memory. */
 int hdgx_set_flags(hdgx_ctx* ctx, int flags);
 const char* hdgx_last_error(hdgx_ctx* ctx);
 
