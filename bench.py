@@ -106,7 +106,7 @@ def cpu_baseline(P, nthreads):
     O.elemental(*args, nthreads=nthreads)                     # warm-up
     runs = []
     out = None
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         out = O.elemental(*args, nthreads=nthreads)
         runs.append(time.perf_counter() - t0)
@@ -117,7 +117,7 @@ def cpu_baseline(P, nthreads):
               P.uh[:, :, :ns], P.sh[:, :, :ns])
     O.elemental(*sample, nthreads=1)
     r1 = []
-    for _ in range(3):
+    for _ in range(5):
         t0 = time.perf_counter()
         O.elemental(*sample, nthreads=1)
         r1.append(time.perf_counter() - t0)
@@ -433,7 +433,7 @@ def main():
                      "flops_per_element": (W.schur_flops if ts >= tq else W.quad_flops)},
         "cpu_baseline": {"value": cpu_rate, "unit": "elements/s", "cores": ncpu, "kind": "port",
                          "sample": f"full workload ({ne} elements), oracle FP64 C++ restatement, "
-                                   f"{build_kind} build, OpenMP over elements, median of 3 after 1 warm-up",
+                                   f"{build_kind} build, OpenMP over elements, median of 5 after 1 warm-up",
                          "single_core_value": cpu_rate1, "single_core_sample": f"first {min(2048, ne)} elements"},
         "speedup_vs_cpu": value / cpu_rate,
         "residual": resid,
