@@ -782,17 +782,23 @@ HDGX_DEV void schur_mfma_body(const SchurArgs& A, const int32_t* __restrict__ pe
           dprev = dmx;
         }
       }
-      // DUDG / DUDG_DUH columns of this tile (coalesced over rows)
-      for (int cc = 0; cc < 16; ++cc) {
-        const int q = q0 + cc;
-        if (q >= NRHS) break;
-        for (int r = lane; r < LDO; r += 64) {
-          const double v = sXt[r + NCOLP * cc];
-          if (!isfinite(v)) st |= 4;
-          if (q == 0) A.dudg[e * LDO + r] = v;
-          else Xe[r + (int64_t)LDO * rhs_t<C>(q)] = v;
+      // DUDG / DUDG_DUH columns of this tile (coalesced over rows); the
+      // rows kept are all NCOL (the usual case, a compile-time bound) or the
+      // first LDO
+      auto put_cols = [&](const int ldo, const int nrow) {
+        for (int cc = 0; cc < 16; ++cc) {
+          const int q = q0 + cc;
+          if (q >= NRHS) break;
+          for (int r = lane; r < nrow; r += 64) {
+            const double v = sXt[r + NCOLP * cc];
+            if (!isfinite(v)) st |= 4;
+            if (q == 0) A.dudg[e * ldo + r] = v;
+            else Xe[r + (int64_t)ldo * rhs_t<C>(q)] = v;
+          }
         }
-      }
+      };
+      if (LDO == NCOL) put_cols(NCOL, NCOL);
+      else put_cols(LDO, LDO);
       // AE(:, tile) = H + GK X, FE = Rh - GK x (schur_uq.c:279-285); all AE row
       // tiles advance together, fragments fetched a batch of k-steps ahead
       {
