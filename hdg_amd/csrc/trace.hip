@@ -974,6 +974,29 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
   return HDGX_OK;
 }
 
+int hdgx_color_faces(int nbf, int64_t nf, const int32_t* f2f, int32_t* color, int32_t* ncolors) {
+  if (nbf < 1 || nf < 0 || !ncolors || (nf > 0 && (!f2f || !color))) return HDGX_ERR_ARG;
+  // greedy in face order: the smallest colour none of the already coloured
+  // neighbours has (a face has nbf - 1 neighbours, so nbf colours suffice)
+  std::vector<char> used(nbf + 1);
+  int nc = 0;
+  for (int64_t i = 0; i < nf; ++i) color[i] = -1;
+  for (int64_t i = 0; i < nf; ++i) {
+    std::fill(used.begin(), used.end(), 0);
+    for (int k = 1; k < nbf; ++k) {
+      const int32_t j = f2f[k + (int64_t)nbf * i];
+      if (j >= nf) return HDGX_ERR_ARG;
+      if (j >= 0 && color[j] >= 0) used[color[j]] = 1;
+    }
+    int c = 0;
+    while (used[c]) ++c;
+    color[i] = c;
+    nc = std::max(nc, c + 1);
+  }
+  *ncolors = nc;
+  return HDGX_OK;
+}
+
 int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, const int32_t* elcon, const double* UH,
                       double* UHE) {
   if (!ctx) return HDGX_ERR_ARG;

@@ -63,6 +63,7 @@ def lib():
         L.hdgx_gmres.argtypes = [vp] * 6 + [i32, ctypes.c_double, i32, vp, vp, vp]
         L.hdgx_trace_gather.argtypes = [vp, i32, i32, i32, i64, vp, vp, vp]
         L.hdgx_trace_scatter.argtypes = [vp, vp, vp, vp]
+        L.hdgx_color_faces.argtypes = [i32, i64, vp, vp, vp]
         L.hdgx_local_update.argtypes = [vp, i32, i32, i32, i32, i32, i64, vp, vp, vp, vp, vp]
         L.hdgx_newton_update.argtypes = [vp, i32, i32, i32, i64, i64, ctypes.c_double, vp, vp, vp, vp, vp, vp]
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
@@ -75,7 +76,7 @@ EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision
            "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
-           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
+           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
 
 
 def _torch():

@@ -67,16 +67,14 @@ def color_faces(f2f):
     colour c are color_rows[color_ptr[c]:color_ptr[c+1]] in ascending index,
     and face_rank is each face's position in that elimination order."""
     nbf, nf = f2f.shape
-    color = -np.ones(nf, dtype=np.int64)
-    nbr = f2f[1:].T
-    for i in range(nf):
-        used = color[nbr[i][nbr[i] >= 0]]
-        c = 0
-        used_set = set(used[used >= 0].tolist())
-        while c in used_set:
-            c += 1
-        color[i] = c
-    ncolors = int(color.max()) + 1 if nf else 0
+    f2f32 = np.asfortranarray(np.asarray(f2f, dtype=np.int32))
+    color = np.empty(nf, dtype=np.int32)
+    nc = ctypes.c_int32(0)
+    rc = lib().hdgx_color_faces(nbf, nf, f2f32.ctypes.data_as(ctypes.c_void_p), color.ctypes.data_as(ctypes.c_void_p),
+                                ctypes.byref(nc))
+    if rc:
+        raise HdgxError("hdgx_color_faces: bad face table")
+    ncolors = nc.value
     order = np.lexsort((np.arange(nf), color))
     rank = np.empty(nf, dtype=np.int64)
     rank[order] = np.arange(nf)

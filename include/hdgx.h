@@ -308,6 +308,15 @@ int hdgx_gmres(hdgx_ctx* ctx, const hdgx_trace* T, const double* K, const double
 int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, const int32_t* elcon,
                       const double* UH, double* UHE);
 
+/* Greedy colouring of the face graph for the block-ILU(0) row order
+ * (hdgx_trace.color_ptr / color_rows): face i, in order, takes the smallest
+ * colour none of its neighbours f2f(1:nbf-1, i) (0-based, -1 none) already
+ * has.  Host memory; no context.  The reference orders the rows by minimum
+ * discarded fill (kernel/computeMDFordering.m), which is sequential; a
+ * colour order lets each colour's rows be factored and solved in parallel.
+ *   f2f (nbf, nf) int32, color (nf) int32 out, *ncolors out. */
+int hdgx_color_faces(int nbf, int64_t nf, const int32_t* f2f, int32_t* color, int32_t* ncolors);
+
 /* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace
  * residual of kernel/hdg_residual.m:41-52 (sparse(il, 1, RH)), summed per face
  * from its one or two elements (no atomics, the same bits every run).

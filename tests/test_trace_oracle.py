@@ -217,3 +217,37 @@ def test_uniform_state_gives_a_zero_update(oracle):
     norms, udg2, UH2 = _newton(oracle, P, udg, UH, iters=1)
     assert norms[0] < 1e-10
     assert np.abs(UH2 - UH).max() < 1e-10 and np.abs(udg2 - udg).max() < 1e-10
+
+
+def _greedy_colours(f2f):
+    """The greedy colouring spelled out: face i takes the smallest colour that
+    none of its already coloured neighbours has."""
+    nbf, nf = f2f.shape
+    color = -np.ones(nf, dtype=np.int64)
+    for i in range(nf):
+        used = {int(color[j]) for j in f2f[1:, i] if j >= 0 and color[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        color[i] = c
+    return color
+
+
+@pytest.mark.parametrize("nd,p,n", [(2, 1, 6), (2, 3, 5), (3, 2, 2)])
+def test_face_colouring(nd, p, n):
+    from hdg_amd.master import mkmaster
+    from hdg_amd.mesh import mkmesh_cube, mkmesh_square
+    from hdg_amd.runtime import HdgxError
+    from hdg_amd.trace import check_coloring, color_faces, mkf2f
+    m = mkmaster(nd, p)
+    mesh = mkmesh_square(n, n, m, jitter_amp=0.1, seed=n) if nd == 2 else mkmesh_cube(n, m, jitter_amp=0.1, seed=n)
+    f2f = mkf2f(mesh)
+    ptr, rows, rank, color = color_faces(f2f)
+    assert np.array_equal(color, _greedy_colours(f2f))
+    assert check_coloring(f2f, color)
+    assert ptr[-1] == f2f.shape[1] and np.array_equal(rows[rank], np.arange(f2f.shape[1]))
+    assert np.all(np.diff(color[rows]) >= 0)
+    bad = f2f.copy()
+    bad[1, 0] = f2f.shape[1]
+    with pytest.raises(HdgxError):
+        color_faces(bad)
