@@ -725,7 +725,7 @@ __global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(3)
 
 // residual only: Ru, Rh (hdg_residual.m's sign), Rq
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) void resid_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+__global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void resid_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C, true> lds;
   quad_body<C, true>(A, T, ap, blockIdx.x, lds);
 }
