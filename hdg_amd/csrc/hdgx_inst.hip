@@ -6,6 +6,7 @@
 #include "quad_kernel.hip"
 #include "getq_kernel.hip"
 #include "schur_mfma.hip"
+#include "schur_lds.hip"
 #include "seg_kernels.hip"
 
 namespace hdgx {
@@ -30,6 +31,12 @@ hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
 
 template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
+  if constexpr (SchurLdsShape<C>::USE) {
+    if (a.prec == 0) {
+      hipLaunchKernelGGL(schur_lds_kernel<C>, dim3((unsigned)a.ne), dim3(SCHUR_LDS_THREADS), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if (a.prec == 1)
     hipLaunchKernelGGL((schur_mfma_ir_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
   else
