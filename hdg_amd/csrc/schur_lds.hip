@@ -1,5 +1,5 @@
-// u-only FP64 elimination for the large local systems (four or more 16-row
-// tiles: n_u > 48, i.e. Euler p = 4 triangles and p = 2 tetrahedra),
+// u-only FP64 elimination for the large local systems with four 16-row tiles
+// and n_u <= 56 (Euler p = 2 tetrahedra, n_u = 50),
 // kernel/schur_u.c:42-80 and its mexFunction's AE/FE (:143-161):
 //   [du | dlu] = D \ [Ru | -F],  AE = H + GK dlu,  FE = Rh - GK du.
 // The MFMA form holds [D | RHS] in accumulator tiles, which at four row tiles
@@ -29,7 +29,11 @@ struct SchurLdsShape {
   static constexpr int GR = 8;                        // GK rows per LDS pass (16 spills)
   static constexpr int NU = C::NU, NH = C::NH, LDA = NU | 1;
   static constexpr int NRHS = 1 + NH;
-  static constexpr bool USE = !C::UQ && NU > 48 && NRHS <= SCHUR_LDS_THREADS;
+  // n_u <= 56: the solution column (2 n_u VGPRs) leaves room for three
+  // wavefronts per SIMD; at n_u = 60 (Euler p = 4 triangles) the solve would
+  // spill (168 VGPRs' worth at one wavefront per SIMD) and those systems stay
+  // on the MFMA form
+  static constexpr bool USE = !C::UQ && NU > 48 && NU <= 56 && NRHS <= SCHUR_LDS_THREADS;
 };
 
 // the larger |value|, the smaller row on ties (idamax's choice)
