@@ -4,8 +4,10 @@
 //   [du | dlu] = D \ [Ru | -F],  AE = H + GK dlu,  FE = Rh - GK du.
 // The MFMA form holds [D | RHS] in accumulator tiles, which at four row tiles
 // needs 256 VGPRs plus 256 AGPRs: one wavefront per SIMD, latency exposed.
-// Here one 256-thread workgroup per element keeps D in LDS and several
-// elements share a CU:
+// Here one 128-thread workgroup (two wavefronts) per element keeps D in LDS
+// and four elements share a CU (243 VGPRs, two wavefronts per SIMD; its phases
+// are latency chains, so elements in flight set the rate -- the 256-thread
+// form held three at 167 VGPRs and was 4 % slower):
 //   1. D -> LDS; LU with partial pivoting (dgetf2's order: the first largest
 //      |.| of the column, whole rows interchanged), three barriers per column;
 //   2. thread t < 1 + n_h takes RHS column t (Ru, then -F's columns) into
@@ -22,17 +24,16 @@
 
 namespace hdgx {
 
-constexpr int SCHUR_LDS_THREADS = 256;
+constexpr int SCHUR_LDS_THREADS = 128;   // two wavefronts: the 121 RHS columns of tetrahedra p2
 
 template <class C>
 struct SchurLdsShape {
   static constexpr int GR = 8;                        // GK rows per LDS pass (16 spills)
   static constexpr int NU = C::NU, NH = C::NH, LDA = NU | 1;
   static constexpr int NRHS = 1 + NH;
-  // n_u <= 56: the solution column (2 n_u VGPRs) leaves room for three
-  // wavefronts per SIMD; at n_u = 60 (Euler p = 4 triangles) the solve would
-  // spill (168 VGPRs' worth at one wavefront per SIMD) and those systems stay
-  // on the MFMA form
+  // n_u <= 56: the solution column (2 n_u VGPRs) fits; at n_u = 60 (Euler
+  // p = 4 triangles) the solve spills at one wavefront per SIMD and those
+  // systems stay on the MFMA form
   static constexpr bool USE = !C::UQ && NU > 48 && NU <= 56 && NRHS <= SCHUR_LDS_THREADS;
 };
 
