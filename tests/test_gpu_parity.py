@@ -141,3 +141,22 @@ def test_singular_element_status(ctx, oracle):
     ref = oracle.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)[4]
     assert st[0] != 0 and np.all(st[1:] == 0)
     assert (st[0] != 0) == (ref[0] != 0)
+
+
+def test_singular_element_status_tetrahedra(ctx, oracle):
+    """Tetrahedra p2 (n_u = 50, the LDS elimination): a singular D in one
+    element sets that element's zero-pivot bit and leaves every other
+    element's outputs bit for bit as they were."""
+    P = _inputs("euler3d_p2")
+    print("[gpu] singular D, tetrahedra p2", flush=True)
+    mats = oracle.elemental_matrices(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    keys = ("BD", "F", "GK", "H", "Ru", "Rh")
+    ref = [o.numpy() for o in ctx.schur_u(*(mats[k] for k in keys))]
+    BD = np.asfortranarray(mats["BD"].copy())
+    BD[:, :, 1, 0, 0] = 0.0                    # column (node 1, component 0) of element 0's D
+    got = [o.numpy() for o in ctx.schur_u(BD, *(mats[k] for k in keys[1:]))]
+    st, st_ref = got[4], ref[4]
+    assert np.all(st_ref == 0)
+    assert st[0] & 2 and np.all(st[1:] == 0)
+    for g, r in zip(got[:4], ref[:4]):
+        assert np.array_equal(g[..., 1:], r[..., 1:])
