@@ -68,7 +68,7 @@ def build(changed_only=False, jobs=None):
         cmd = [HIPCC, *FLAGS, f"-DHDGX_PHYS={ph}", f"-DHDGX_ND={nd}", "-c", os.path.join(CSRC, "mixed_inst.hip"),
                "-o", obj]
         jobs_list.append((obj, cmd))
-    for name in ("hdgx_api", "trace", "mex_inplace"):
+    for name in ("hdgx_api", "trace", "mex_inplace", "mesh"):
         obj = os.path.join(BUILD, f"{name}.o")
         jobs_list.append((obj, [HIPCC, *FLAGS, "-c", os.path.join(CSRC, f"{name}.hip"), "-o", obj]))
     todo = [(o, c) for o, c in jobs_list if not changed_only or _stale(o)]

@@ -67,6 +67,8 @@ def lib():
         L.hdgx_local_update.argtypes = [vp, i32, i32, i32, i32, i32, i64, vp, vp, vp, vp, vp]
         L.hdgx_newton_update.argtypes = [vp, i32, i32, i32, i64, i64, ctypes.c_double, vp, vp, vp, vp, vp, vp]
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
+        L.hdgx_mesh_faces.argtypes = [vp, i32, i64, i32, vp, vp, vp, vp]
+        L.hdgx_mesh_elcon.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -76,7 +78,8 @@ EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision
            "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
-           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2"]
+           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2",
+           "hdgx_mesh_faces", "hdgx_mesh_elcon"]
 
 
 def _torch():
@@ -293,6 +296,37 @@ class Context:
         self._check(lib().hdgx_residual(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr, h.ptr,
                                         _p(s), Ru.ptr, Rh.ptr, _p(Rq)))
         return Ru, Rh, Rq
+
+    def mesh_faces(self, t, nvert=None):
+        """Faces on the device (mesh/mkmesh/mkt2f.m, 0-based): t (ne, nd+1)
+        vertex ids.  Returns (f, t2f, nf) with f a Dev (nd+2, nf) -- column i
+        is mesh.f's row i before boundary tagging -- and t2f a Dev (nd+1, ne),
+        mesh.t2f transposed."""
+        t = np.asarray(t)
+        ne, nv = t.shape
+        nd = nv - 1
+        nvert = int(t.max()) + 1 if nvert is None else int(nvert)
+        T = Dev.from_numpy(t.T, np.int32)
+        t2f = Dev.empty((nv, ne), "int32")
+        f = Dev.empty((nd + 2, ne * nv), "int32")
+        nf = ctypes.c_int64(0)
+        self._check(lib().hdgx_mesh_faces(self.h, nd, ne, nvert, T.ptr, t2f.ptr, f.ptr, ctypes.byref(nf)))
+        n = nf.value
+        return Dev(f.t[: (nd + 2) * max(n, 1)], (nd + 2, n)), t2f, n
+
+    def mesh_elcon(self, dgnodes, perm, f, t2f, nf):
+        """Trace-node numbering on the device (elconnectivities.m:147-212):
+        f, t2f, nf as mesh_faces gives them.  Returns elcon, a Dev
+        (npf*nfe, ne) equal to mesh.elcon."""
+        dg = _dev(dgnodes)
+        npv, ncd, ne = dg.shape
+        perm = np.asarray(perm)
+        npf, nfe = perm.shape
+        P = Dev.from_numpy(perm, np.int32)
+        out = Dev.empty((npf * nfe, ne), "int32")
+        self._check(lib().hdgx_mesh_elcon(self.h, nfe - 1, npv, ncd, npf, ne, nf, dg.ptr, P.ptr, f.ptr, t2f.ptr,
+                                          out.ptr))
+        return out
 
     def getq(self, master, dgnodes, udg, uh, sh=None, fc_q=1.0, ncu=None, out=None, inplace=False, shapnv=None):
         """QDG = getq(master, mesh, UDG, UH, SH, fc_q) (kernel/getq.m:1-129) for the

@@ -317,6 +317,32 @@ int hdgx_trace_gather(hdgx_ctx* ctx, int ncu, int npf, int nfe, int64_t ne, cons
  *   f2f (nbf, nf) int32, color (nf) int32 out, *ncolors out. */
 int hdgx_color_faces(int nbf, int64_t nf, const int32_t* f2f, int32_t* color, int32_t* ncolors);
 
+/* Mesh faces on the device (mesh/mkmesh/mkt2f.m): faces numbered in order of
+ * first visit over (element e, local face j), 0-based.
+ *   t (nd+1, ne) int32 vertex ids 0..nvert-1 (3D: nvert <= 2^21);
+ *   t2f (nd+1, ne) int32 out: face of local face j of element e;
+ *   f (nd+2, ne*(nd+1)) int32 out, column i = face i for i < *nf: its nd
+ *     vertices as its first element lists them (the local face tables of
+ *     mkt2f.m), the first element, the second element or -1 on the boundary;
+ *   *nf out (host) the number of faces.
+ * The reference's f and t2f are 1-based and f's last column later holds -bnd
+ * ids (setbndnbrs); these are 0-based and keep -1.  Device arrays; the call
+ * synchronises the context's stream.  HDGX_ERR_ARG for a vertex id out of
+ * range or a face shared by more than two elements. */
+int hdgx_mesh_faces(hdgx_ctx* ctx, int nd, int64_t ne, int32_t nvert, const int32_t* t, int32_t* t2f, int32_t* f,
+                    int64_t* nf);
+
+/* Trace-node numbering on the device (mesh/mkmesh/elconnectivities.m:147-212):
+ * face i owns trace nodes i*npf .. i*npf+npf-1 in its first element's order;
+ * each node of the second element's face takes the first-side node at the
+ * least squared distance over the first min(ncd, 3) coordinates (an error
+ * past 1e-16).
+ *   dgnodes (npv, ncd, ne) FP64; perm (npf, nd+1) int32 (master.perm,
+ *   0-based); f, t2f as hdgx_mesh_faces gives them; elcon (npf*(nd+1), ne)
+ *   int32 out.  Device arrays; synchronises the context's stream. */
+int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne, int64_t nf, const double* dgnodes,
+                    const int32_t* perm, const int32_t* f, const int32_t* t2f, int32_t* elcon);
+
 /* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace
  * residual of kernel/hdg_residual.m:41-52 (sparse(il, 1, RH)), summed per face
  * from its one or two elements (no atomics, the same bits every run).

@@ -1,0 +1,79 @@
+"""Mesh connectivity on the device (hdgx_mesh_faces, hdgx_mesh_elcon) equals
+the host construction of hdg_amd/mesh.py (mesh/mkmesh/mkt2f.m,
+elconnectivities.m:147-212) integer for integer: square and cube meshes,
+jittered and re-oriented, one element, no element, and the error paths
+(a face of three elements, a vertex id out of range, trace nodes that do not
+meet)."""
+import numpy as np
+import pytest
+
+from tests.conftest import has_gpu
+
+pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
+
+from hdg_amd.master import mkmaster  # noqa: E402
+from hdg_amd.mesh import _orient, createnodes, cubemesh, hdgelcon, jitter, mkt2f, squaremesh  # noqa: E402
+from hdg_amd.runtime import HdgxError  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    from hdg_amd.runtime import Context
+    return Context(0)
+
+
+def _check(ctx, p, t, master):
+    nd = p.shape[1]
+    f_ref, t2f_ref = mkt2f(t, nd)
+    f, t2f, nf = ctx.mesh_faces(t)
+    assert nf == f_ref.shape[0]
+    assert np.array_equal(f.numpy().T, f_ref)
+    assert np.array_equal(t2f.numpy().T, t2f_ref)
+    dg = createnodes(p, t, master.plocvl)
+    el_ref, nsiz = hdgelcon(dg, master.perm, f_ref, t2f_ref)
+    el = ctx.mesh_elcon(dg, master.perm, f, t2f, nf)
+    assert nsiz == nf * master.perm.shape[0]
+    assert np.array_equal(el.numpy(), el_ref)
+
+
+@pytest.mark.parametrize("nx,ny,porder,amp", [(7, 5, 3, 0.2), (64, 64, 2, 0.0), (200, 150, 4, 0.1)])
+def test_square(ctx, nx, ny, porder, amp):
+    print(f"[gpu] device mesh: square {nx}x{ny} p{porder}", flush=True)
+    p, t = squaremesh(nx + 1, ny + 1)
+    if amp:
+        p = jitter(p, 1.0 / max(nx, ny), amp, seed=nx)
+    _check(ctx, p, t, mkmaster(2, porder))
+
+
+@pytest.mark.parametrize("n,porder", [(3, 2), (12, 1), (16, 2)])
+def test_cube(ctx, n, porder):
+    print(f"[gpu] device mesh: cube {n} p{porder}", flush=True)
+    p, t = cubemesh(n + 1, n + 1, n + 1)
+    p = jitter(p, 1.0 / n, 0.2, seed=n)
+    t = _orient(p, t)
+    _check(ctx, p, t, mkmaster(3, porder))
+
+
+def test_one_and_no_element(ctx):
+    print("[gpu] device mesh: one triangle, one tetrahedron, empty", flush=True)
+    _check(ctx, np.array([[0.0, 0.0], [1.0, 0.0], [0.0, 1.0]]), np.array([[0, 1, 2]]), mkmaster(2, 2))
+    _check(ctx, np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]), np.array([[0, 1, 2, 3]]), mkmaster(3, 2))
+    f, t2f, nf = ctx.mesh_faces(np.zeros((0, 3), dtype=np.int64), nvert=0)
+    assert nf == 0
+
+
+def test_errors(ctx):
+    print("[gpu] device mesh: error paths", flush=True)
+    # three triangles on the edge (0, 1)
+    with pytest.raises(HdgxError, match="more than two"):
+        ctx.mesh_faces(np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4]]))
+    with pytest.raises(HdgxError, match="vertex id"):
+        ctx.mesh_faces(np.array([[0, 1, 5]]), nvert=3)
+    # the second element's copy of the shared face nodes moved off the first's
+    m = mkmaster(2, 2)
+    p, t = squaremesh(3, 3)
+    f, t2f, nf = ctx.mesh_faces(t)
+    dg = createnodes(p, t, m.plocvl)
+    dg[:, :, 1] += 1e-6
+    with pytest.raises(HdgxError, match="do not match"):
+        ctx.mesh_elcon(dg, m.perm, f, t2f, nf)
