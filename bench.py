@@ -182,6 +182,39 @@ def getq_leg(torch, ctx, P, dev, steps, warmup):
             "rq_over_rq_at_q0": red}
 
 
+def mesh_leg(torch, ctx, n=256, porder=3, reps=3):
+    """Mesh connectivity for a new 2 n^2-triangle p3 mesh: hdgx_mesh_faces +
+    hdgx_mesh_elcon (vertex table and DG nodes already on the device) against
+    the host construction of hdg_amd/mesh.py (mkt2f, hdgelcon), whose arrays
+    they must equal."""
+    from hdg_amd.master import mkmaster
+    from hdg_amd.mesh import createnodes, hdgelcon, jitter, mkt2f, squaremesh
+    from hdg_amd.runtime import Dev
+    m = mkmaster(2, porder)
+    p, t = squaremesh(n + 1, n + 1)
+    p = jitter(p, 1.0 / n, 0.2, seed=n)
+    dg = Dev.from_numpy(createnodes(p, t, m.plocvl))
+    ctx.mesh_faces(t)                                  # warm-up (module load)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        f, t2f, nf = ctx.mesh_faces(t)
+        el = ctx.mesh_elcon(dg, m.perm, f, t2f, nf)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    f_ref, t2f_ref = mkt2f(t, 2)
+    el_ref, _ = hdgelcon(dg.numpy(), m.perm, f_ref, t2f_ref)
+    dth = time.perf_counter() - t0
+    same = bool(np.array_equal(f.numpy().T, f_ref) and np.array_equal(t2f.numpy().T, t2f_ref)
+                and np.array_equal(el.numpy(), el_ref))
+    ne = t.shape[0]
+    log(f"[bench] mesh: {ne} elements, {nf} faces: device {dt * 1e3:.2f} ms, host {dth * 1e3:.1f} ms, equal {same}")
+    return {"workload": f"mkt2f + elcon, 2x{n}x{n} triangles p{porder}", "elements": ne, "faces": nf,
+            "ms_device": dt * 1e3, "ms_host_numpy": dth * 1e3, "equal_to_host": same,
+            "note": "device time includes the vertex-table upload and the two host syncs"}
+
+
 def host_leg(torch, ctx, P, reps=2):
     """hdgx_elemental on host (pageable numpy) arrays, HDGX_HOST_PTRS: the
     rate a caller handing over host buffers sees, transfers included."""
@@ -386,6 +419,7 @@ def main():
     resid = residual_leg(torch, ctx, P, dev, a.steps, a.warmup)
     getq = getq_leg(torch, ctx, P, dev, a.steps, a.warmup)
     host = host_leg(torch, ctx, P)
+    mesh = mesh_leg(torch, ctx)
 
     ncpu = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     ref, cpu_rate, cpu_rate1, build_kind, cpu_secs, noise_floor = cpu_baseline(P, ncpu)
@@ -439,6 +473,7 @@ def main():
         "residual": resid,
         "getq": getq,
         "host_buffers": host,
+        "mesh_preprocess": mesh,
     }
     del out, dev
     torch.cuda.empty_cache()
