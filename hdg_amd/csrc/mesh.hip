@@ -6,7 +6,9 @@
 //   hdgx_mesh_elcon  mesh/mkmesh/elconnectivities.m:147-212 -- face i owns
 //                    trace nodes i*npf .. i*npf+npf-1 in its first element's
 //                    order; the second element's face nodes are matched to
-//                    them by coordinates.
+//                    them by coordinates;
+//   hdgx_mesh_f2f    mesh/mkmesh/mkf2f.m:8-55 -- each face's neighbouring
+//                    faces, the block pattern of the trace system.
 // Faces are found without a sort: every (element, local face) inserts its
 // sorted vertex tuple into an open-addressing hash table (linear probing,
 // capacity a power of two at least twice the number of local faces), keeping
@@ -211,6 +213,39 @@ __global__ void elcon_kernel(int ncmp, int npv, int ncd, int npf, int nfe, int64
   elcon[b + npf * j2 + base * e2] = (int32_t)(i * npf + ab);
 }
 
+// one thread per face: fconn (4, nf) = [e1, j1, e2 or -1, j2 or 0] and the
+// face-to-face table f2f (2 nfe - 1, nf): the face, the other faces of its
+// first element, then of its second (ascending local index), -1 where none
+__global__ void f2f_kernel(int nd, int64_t nf, const int32_t* __restrict__ f, const int32_t* __restrict__ t2f,
+                           int32_t* fconn, int32_t* f2f, int* err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int nvf = nd, nfe = nd + 1, nbf = 2 * nfe - 1;
+  const int32_t* row = f + (int64_t)(nvf + 2) * i;
+  const int32_t e1 = row[nvf];
+  const int32_t e2 = row[nvf + 1] >= 0 ? row[nvf + 1] : -1;
+  int j1 = -1, j2 = -1;
+  for (int j = 0; j < nfe; ++j) {
+    if (j1 < 0 && t2f[j + (int64_t)nfe * e1] == i) j1 = j;
+    if (e2 >= 0 && j2 < 0 && t2f[j + (int64_t)nfe * e2] == i) j2 = j;
+  }
+  if (j1 < 0 || (e2 >= 0 && j2 < 0)) { atomicOr(err, 4); return; }
+  if (fconn) {
+    fconn[4 * i] = e1;
+    fconn[4 * i + 1] = j1;
+    fconn[4 * i + 2] = e2;
+    fconn[4 * i + 3] = e2 >= 0 ? j2 : 0;
+  }
+  int32_t* out = f2f + (int64_t)nbf * i;
+  out[0] = (int32_t)i;
+  for (int k = 1; k < nfe; ++k) {
+    const int l1 = k - 1 < j1 ? k - 1 : k;
+    out[k] = t2f[l1 + (int64_t)nfe * e1];
+    const int l2 = k - 1 < j2 ? k - 1 : k;
+    out[nfe - 1 + k] = e2 >= 0 ? t2f[l2 + (int64_t)nfe * e2] : -1;
+  }
+}
+
 // device scratch of one call, released before returning
 struct Scratch {
   void* p = nullptr;
@@ -299,5 +334,26 @@ int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne
   HIP_TRY(ctx, hipStreamSynchronize(s));
   if (herr & 4) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: a face missing from its element's t2f");
   if (herr & 8) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: trace nodes do not match across a face");
+  return HDGX_OK;
+}
+
+int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
+                  int32_t* f2f) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if ((nd != 2 && nd != 3) || nf < 0) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: sizes");
+  if (nf == 0) return HDGX_OK;
+  if (!f || !t2f || !f2f) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  Scratch S;
+  HIP_TRY(ctx, hipMalloc(&S.p, sizeof(int)));
+  int* err = static_cast<int*>(S.p);
+  hipStream_t s = ctx->stream;
+  HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
+  hipLaunchKernelGGL(f2f_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nd, nf, f, t2f, fconn, f2f, err);
+  HIP_TRY(ctx, hipGetLastError());
+  int herr = 0;
+  HIP_TRY(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (herr) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: a face missing from its element's t2f");
   return HDGX_OK;
 }

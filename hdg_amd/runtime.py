@@ -69,6 +69,7 @@ def lib():
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
         L.hdgx_mesh_faces.argtypes = [vp, i32, i64, i32, vp, vp, vp, vp]
         L.hdgx_mesh_elcon.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, vp, vp]
+        L.hdgx_mesh_f2f.argtypes = [vp, i32, i64, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -79,7 +80,7 @@ EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2",
-           "hdgx_mesh_faces", "hdgx_mesh_elcon"]
+           "hdgx_mesh_faces", "hdgx_mesh_elcon", "hdgx_mesh_f2f"]
 
 
 def _torch():
@@ -327,6 +328,17 @@ class Context:
         self._check(lib().hdgx_mesh_elcon(self.h, nfe - 1, npv, ncd, npf, ne, nf, dg.ptr, P.ptr, f.ptr, t2f.ptr,
                                           out.ptr))
         return out
+
+    def mesh_f2f(self, f, t2f, nf):
+        """Face-to-face table and face connectivity on the device (mkf2f.m):
+        f, t2f, nf as mesh_faces gives them.  Returns (fconn (4, nf),
+        f2f (2 nfe - 1, nf)) Dev arrays, equal to hdg_amd.trace's
+        face_connectivity and mkf2f."""
+        nd = f.shape[0] - 2
+        fconn = Dev.empty((4, nf), "int32")
+        f2f = Dev.empty((2 * nd + 1, nf), "int32")
+        self._check(lib().hdgx_mesh_f2f(self.h, nd, nf, f.ptr, t2f.ptr, fconn.ptr, f2f.ptr))
+        return fconn, f2f
 
     def getq(self, master, dgnodes, udg, uh, sh=None, fc_q=1.0, ncu=None, out=None, inplace=False, shapnv=None):
         """QDG = getq(master, mesh, UDG, UH, SH, fc_q) (kernel/getq.m:1-129) for the

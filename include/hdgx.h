@@ -343,6 +343,16 @@ int hdgx_mesh_faces(hdgx_ctx* ctx, int nd, int64_t ne, int32_t nvert, const int3
 int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne, int64_t nf, const double* dgnodes,
                     const int32_t* perm, const int32_t* f, const int32_t* t2f, int32_t* elcon);
 
+/* Face-to-face table on the device (mesh/mkmesh/mkf2f.m:8-55, 0-based):
+ * f2f (2nd+1, nf) int32 out: the face, the other faces of its first element,
+ * then of its second, in ascending local index, -1 where there is no second
+ * element; fconn (4, nf) int32 out (may be NULL): first element, its local
+ * face, second element or -1, its local face or 0 (hdgx_trace.fconn).
+ * f, t2f as hdgx_mesh_faces gives them (a negative second element, e.g. a
+ * boundary tag, means none).  Device arrays; synchronises the stream. */
+int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
+                  int32_t* f2f);
+
 /* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace
  * residual of kernel/hdg_residual.m:41-52 (sparse(il, 1, RH)), summed per face
  * from its one or two elements (no atomics, the same bits every run).

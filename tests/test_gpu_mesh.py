@@ -1,9 +1,12 @@
-"""Mesh connectivity on the device (hdgx_mesh_faces, hdgx_mesh_elcon) equals
-the host construction of hdg_amd/mesh.py (mesh/mkmesh/mkt2f.m,
-elconnectivities.m:147-212) integer for integer: square and cube meshes,
+"""Mesh connectivity on the device (hdgx_mesh_faces, hdgx_mesh_elcon,
+hdgx_mesh_f2f) equals the host construction of hdg_amd/mesh.py and
+hdg_amd/trace.py (mesh/mkmesh/mkt2f.m, elconnectivities.m:147-212, mkf2f.m)
+integer for integer: square and cube meshes,
 jittered and re-oriented, one element, no element, and the error paths
 (a face of three elements, a vertex id out of range, trace nodes that do not
 meet)."""
+from types import SimpleNamespace
+
 import numpy as np
 import pytest
 
@@ -14,6 +17,7 @@ pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a
 from hdg_amd.master import mkmaster  # noqa: E402
 from hdg_amd.mesh import _orient, createnodes, cubemesh, hdgelcon, jitter, mkt2f, squaremesh  # noqa: E402
 from hdg_amd.runtime import HdgxError  # noqa: E402
+from hdg_amd.trace import face_connectivity, mkf2f  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -34,6 +38,10 @@ def _check(ctx, p, t, master):
     el = ctx.mesh_elcon(dg, master.perm, f, t2f, nf)
     assert nsiz == nf * master.perm.shape[0]
     assert np.array_equal(el.numpy(), el_ref)
+    ms = SimpleNamespace(f=f_ref, t2f=t2f_ref)
+    fconn, f2f = ctx.mesh_f2f(f, t2f, nf)
+    assert np.array_equal(fconn.numpy(), face_connectivity(ms))
+    assert np.array_equal(f2f.numpy(), mkf2f(ms))
 
 
 @pytest.mark.parametrize("nx,ny,porder,amp", [(7, 5, 3, 0.2), (64, 64, 2, 0.0), (200, 150, 4, 0.1)])
