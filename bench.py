@@ -183,13 +183,15 @@ def getq_leg(torch, ctx, P, dev, steps, warmup):
 
 
 def mesh_leg(torch, ctx, n=256, porder=3, reps=3):
-    """Mesh connectivity for a new 2 n^2-triangle p3 mesh: hdgx_mesh_faces +
-    hdgx_mesh_elcon (vertex table and DG nodes already on the device) against
-    the host construction of hdg_amd/mesh.py (mkt2f, hdgelcon), whose arrays
-    they must equal."""
+    """Mesh connectivity for a new 2 n^2-triangle p3 mesh: hdgx_mesh_faces,
+    hdgx_mesh_elcon and hdgx_mesh_f2f (DG nodes already on the device) against
+    the host construction of hdg_amd/mesh.py and hdg_amd/trace.py (mkt2f,
+    hdgelcon, mkf2f), whose arrays they must equal."""
     from hdg_amd.master import mkmaster
     from hdg_amd.mesh import createnodes, hdgelcon, jitter, mkt2f, squaremesh
+    from types import SimpleNamespace
     from hdg_amd.runtime import Dev
+    from hdg_amd.trace import mkf2f
     m = mkmaster(2, porder)
     p, t = squaremesh(n + 1, n + 1)
     p = jitter(p, 1.0 / n, 0.2, seed=n)
@@ -200,17 +202,19 @@ def mesh_leg(torch, ctx, n=256, porder=3, reps=3):
     for _ in range(reps):
         f, t2f, nf = ctx.mesh_faces(t)
         el = ctx.mesh_elcon(dg, m.perm, f, t2f, nf)
+        fconn, f2f = ctx.mesh_f2f(f, t2f, nf)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / reps
     t0 = time.perf_counter()
     f_ref, t2f_ref = mkt2f(t, 2)
     el_ref, _ = hdgelcon(dg.numpy(), m.perm, f_ref, t2f_ref)
+    f2f_ref = mkf2f(SimpleNamespace(f=f_ref, t2f=t2f_ref))
     dth = time.perf_counter() - t0
     same = bool(np.array_equal(f.numpy().T, f_ref) and np.array_equal(t2f.numpy().T, t2f_ref)
-                and np.array_equal(el.numpy(), el_ref))
+                and np.array_equal(el.numpy(), el_ref) and np.array_equal(f2f.numpy(), f2f_ref))
     ne = t.shape[0]
     log(f"[bench] mesh: {ne} elements, {nf} faces: device {dt * 1e3:.2f} ms, host {dth * 1e3:.1f} ms, equal {same}")
-    return {"workload": f"mkt2f + elcon, 2x{n}x{n} triangles p{porder}", "elements": ne, "faces": nf,
+    return {"workload": f"mkt2f + elcon + mkf2f, 2x{n}x{n} triangles p{porder}", "elements": ne, "faces": nf,
             "ms_device": dt * 1e3, "ms_host_numpy": dth * 1e3, "equal_to_host": same,
             "note": "device time includes the vertex-table upload and the two host syncs"}
 
