@@ -297,6 +297,7 @@ int hdgx_destroy(hdgx_ctx* ctx) {
   if (ctx->tws) (void)hipFree(ctx->tws);
   if (ctx->segdev) (void)hipFree(ctx->segdev);
   if (ctx->hws) (void)hipFree(ctx->hws);
+  if (ctx->mws) (void)hipFree(ctx->mws);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   delete ctx;
   return HDGX_OK;

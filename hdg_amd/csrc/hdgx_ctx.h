@@ -33,6 +33,8 @@ struct hdgx_ctx {
   size_t segdev_bytes = 0;
   void* hws = nullptr;           // device copies of host arrays (HDGX_HOST_PTRS)
   size_t hws_bytes = 0;
+  void* mws = nullptr;           // mesh-connectivity scratch (hash table, scan)
+  size_t mws_bytes = 0;
 };
 
 namespace hdgx_detail {

@@ -159,6 +159,7 @@ __global__ void face_rows_kernel(int nd, int64_t ne, const int32_t* __restrict__
   const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= ne * nfe) return;
   const int32_t h = slot[q];
+  if (h < 0) return;                               // vertex id out of range: the call fails
   const int32_t q1 = first[h];
   const int32_t id = prefix[q1];
   const int64_t e = q / nfe;
@@ -246,13 +247,16 @@ __global__ void f2f_kernel(int nd, int64_t nf, const int32_t* __restrict__ f, co
   }
 }
 
-// device scratch of one call, released before returning
-struct Scratch {
-  void* p = nullptr;
-  ~Scratch() {
-    if (p) (void)hipFree(p);
-  }
-};
+// the context's mesh scratch, grown on demand and kept for later calls
+int ensure_mws(hdgx_ctx* ctx, size_t bytes) {
+  if (ctx->mws_bytes >= bytes) return HDGX_OK;
+  if (ctx->mws) HIP_TRY(ctx, hipFree(ctx->mws));
+  ctx->mws = nullptr;
+  ctx->mws_bytes = 0;
+  HIP_TRY(ctx, hipMalloc(&ctx->mws, bytes));
+  ctx->mws_bytes = bytes;
+  return HDGX_OK;
+}
 
 }  // namespace
 
@@ -271,33 +275,27 @@ int hdgx_mesh_faces(hdgx_ctx* ctx, int nd, int64_t ne, int32_t nvert, const int3
   unsigned long long cap = 1;
   while (cap < 2ull * (unsigned long long)nq) cap <<= 1;
   const int64_t nb = (nq + SCAN_BLOCK - 1) / SCAN_BLOCK;
-  // keys (cap) | first (cap) | cnt (cap) | slot (nq) | flag (nq) | prefix (nq) | bsum (nb) | grand, err
-  const size_t bytes = cap * 8 + cap * 4 * 2 + (size_t)nq * 4 * 3 + (size_t)nb * 4 + 16 + 64;
-  Scratch S;
-  HIP_TRY(ctx, hipMalloc(&S.p, bytes));
-  char* p = static_cast<char*>(S.p);
-  auto* keys = reinterpret_cast<unsigned long long*>(p);
-  auto* first = reinterpret_cast<int32_t*>(p + cap * 8);
+  // err, grand | keys (cap) | first (cap) | cnt (cap) | slot (nq) | flag (nq) | prefix (nq) | bsum (nb)
+  const size_t bytes = 16 + cap * 8 + cap * 4 * 2 + (size_t)nq * 4 * 3 + (size_t)nb * 4;
+  int rc = ensure_mws(ctx, bytes);
+  if (rc) return rc;
+  char* p = static_cast<char*>(ctx->mws);
+  auto* err = reinterpret_cast<int32_t*>(p);
+  auto* grand = err + 1;
+  auto* keys = reinterpret_cast<unsigned long long*>(p + 16);
+  auto* first = reinterpret_cast<int32_t*>(p + 16 + cap * 8);
   auto* cnt = first + cap;
   auto* slot = cnt + cap;
   auto* flag = slot + nq;
   auto* prefix = flag + nq;
   auto* bsum = prefix + nq;
-  auto* grand = bsum + nb;
-  auto* err = reinterpret_cast<int*>(grand + 1);
   hipStream_t s = ctx->stream;
+  HIP_TRY(ctx, hipMemsetAsync(err, 0, 2 * sizeof(int32_t), s));
   HIP_TRY(ctx, hipMemsetAsync(keys, 0xff, cap * 8, s));
   HIP_TRY(ctx, hipMemsetAsync(first, 0x7f, cap * 4, s));   // 0x7f7f7f7f: above every index
   HIP_TRY(ctx, hipMemsetAsync(cnt, 0, cap * 4, s));
-  HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
   hipLaunchKernelGGL(face_insert_kernel, dim3(grid_of(nq, 256)), dim3(256), 0, s, nd, ne, t, nvert, keys, first, cnt,
                      slot, (unsigned long long)(cap - 1), err);
-  HIP_TRY(ctx, hipGetLastError());
-  int herr = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
-  HIP_TRY(ctx, hipStreamSynchronize(s));
-  if (herr & 1) return fail(ctx, HDGX_ERR_ARG, "mesh_faces: vertex id outside 0..nvert-1");
-  if (herr & 2) return fail(ctx, HDGX_ERR_ARG, "mesh_faces: a face shared by more than two elements");
   hipLaunchKernelGGL(first_flag_kernel, dim3(grid_of(nq, 256)), dim3(256), 0, s, nq, first, slot, flag);
   hipLaunchKernelGGL(scan_blocks_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, nq, flag, prefix, bsum);
   hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, nb, bsum, grand);
@@ -305,10 +303,14 @@ int hdgx_mesh_faces(hdgx_ctx* ctx, int nd, int64_t ne, int32_t nvert, const int3
   hipLaunchKernelGGL(face_rows_kernel, dim3(grid_of(nq, 256)), dim3(256), 0, s, nd, ne, t, first, cnt, slot, prefix,
                      t2f, f);
   HIP_TRY(ctx, hipGetLastError());
-  int32_t hnf = 0;
-  HIP_TRY(ctx, hipMemcpyAsync(&hnf, grand, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  // the error word and the face count in one read-back (a failed call leaves
+  // t2f and f undefined)
+  int32_t h2[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(h2, err, sizeof(h2), hipMemcpyDeviceToHost, s));
   HIP_TRY(ctx, hipStreamSynchronize(s));
-  *nf = hnf;
+  if (h2[0] & 1) return fail(ctx, HDGX_ERR_ARG, "mesh_faces: vertex id outside 0..nvert-1");
+  if (h2[0] & 2) return fail(ctx, HDGX_ERR_ARG, "mesh_faces: a face shared by more than two elements");
+  *nf = h2[1];
   return HDGX_OK;
 }
 
@@ -320,9 +322,9 @@ int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne
   if (nf == 0) return HDGX_OK;
   if (!dgnodes || !perm || !f || !t2f || !elcon) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: null array");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  Scratch S;
-  HIP_TRY(ctx, hipMalloc(&S.p, sizeof(int)));
-  int* err = static_cast<int*>(S.p);
+  int rc = ensure_mws(ctx, 16);
+  if (rc) return rc;
+  int* err = static_cast<int*>(ctx->mws);
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
   const int ncmp = ncd < 3 ? ncd : 3;              // coordinates compared
@@ -344,9 +346,9 @@ int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int
   if (nf == 0) return HDGX_OK;
   if (!f || !t2f || !f2f) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: null array");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  Scratch S;
-  HIP_TRY(ctx, hipMalloc(&S.p, sizeof(int)));
-  int* err = static_cast<int*>(S.p);
+  int rc = ensure_mws(ctx, 16);
+  if (rc) return rc;
+  int* err = static_cast<int*>(ctx->mws);
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
   hipLaunchKernelGGL(f2f_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nd, nf, f, t2f, fconn, f2f, err);
