@@ -25,6 +25,20 @@ using hdgx_detail::fail;
 
 namespace {
 
+// Element matrices pass between the quadrature and elimination kernels through
+// a context workspace of at most WS_CAP bytes (a twelfth of MI355X's 288 GB).
+// Larger batches run in equal chunks: 384 000 p = 2 tetrahedra (231 KB of
+// matrices each) in 4 chunks rather than 14 under a 6 GiB cap, a ragged last
+// one avoided -- each chunk boundary drains the GPU (C4 step 142 -> 134 ms).
+constexpr size_t WS_CAP = (size_t)24 << 30;
+
+int64_t ws_chunk(int64_t n, size_t bytes_per_element) {
+  if (n <= 0) return 1;
+  const int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(WS_CAP / bytes_per_element)));
+  const int64_t nch = (n + cmax - 1) / cmax;
+  return (n + nch - 1) / nch;
+}
+
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
   const unsigned char* c = static_cast<const unsigned char*>(p);
   for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
@@ -579,7 +593,7 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   // per-element workspace for the Schur inputs
   const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
                      (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(ne, (int64_t)((size_t)6 << 30) / (int64_t)(per * 8)));
+  const int64_t chunk = ws_chunk(ne, per * 8);
   if (ne == 0) return HDGX_OK;
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
@@ -664,7 +678,7 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
   HIP_TRY(ctx, hipMemcpyAsync(dapps, apps.data(), sizeof(DevApp) * nseg, hipMemcpyHostToDevice, ctx->stream));
   const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
                      (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
-  const int64_t chunk = std::max<int64_t>(1, std::min<int64_t>(total, (int64_t)((size_t)6 << 30) / (int64_t)(per * 8)));
+  const int64_t chunk = ws_chunk(total, per * 8);
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   for (int64_t g0 = 0; g0 < total; g0 += chunk) {
     const int64_t n = std::min(chunk, total - g0);
