@@ -26,15 +26,14 @@ using hdgx_detail::fail;
 namespace {
 
 // Element matrices pass between the quadrature and elimination kernels through
-// a context workspace of at most WS_CAP bytes (a twelfth of MI355X's 288 GB).
-// Larger batches run in equal chunks: 384 000 p = 2 tetrahedra (231 KB of
-// matrices each) in 4 chunks rather than 14 under a 6 GiB cap, a ragged last
-// one avoided -- each chunk boundary drains the GPU (C4 step 142 -> 134 ms).
-constexpr size_t WS_CAP = (size_t)24 << 30;
-
-int64_t ws_chunk(int64_t n, size_t bytes_per_element) {
+// a context workspace of at most ctx->ws_cap bytes (24 GiB by default, a
+// twelfth of MI355X's 288 GB; hdgx_set_workspace_limit).  Larger batches run
+// in chunks of nearly equal size: 384 000 p = 2 tetrahedra (231 KB of
+// matrices each) in 4 chunks rather than 14 under a 6 GiB cap -- each chunk
+// boundary drains the GPU (tetrahedra step 142 -> 132 ms).
+int64_t ws_chunk(int64_t n, size_t bytes_per_element, size_t cap) {
   if (n <= 0) return 1;
-  const int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(WS_CAP / bytes_per_element)));
+  const int64_t cmax = std::max<int64_t>(1, std::min<int64_t>(n, (int64_t)(cap / bytes_per_element)));
   const int64_t nch = (n + cmax - 1) / cmax;
   return (n + nch - 1) / nch;
 }
@@ -330,6 +329,21 @@ int hdgx_set_precision(hdgx_ctx* ctx, int precision) {
   return HDGX_OK;
 }
 
+int hdgx_set_workspace_limit(hdgx_ctx* ctx, int64_t bytes) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (bytes < 0) return fail(ctx, HDGX_ERR_ARG, "workspace limit < 0");
+  ctx->ws_cap = bytes == 0 ? ((size_t)24 << 30) : (size_t)bytes;
+  if (ctx->ws && ctx->ws_bytes > ctx->ws_cap) {
+    // work already queued on the context's stream may still read it
+    HIP_TRY(ctx, hipSetDevice(ctx->device));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_TRY(ctx, hipFree(ctx->ws));
+    ctx->ws = nullptr;
+    ctx->ws_bytes = 0;
+  }
+  return HDGX_OK;
+}
+
 int hdgx_set_flags(hdgx_ctx* ctx, int flags) {
   if (!ctx) return HDGX_ERR_ARG;
   if (flags & ~(HDGX_INPLACE | HDGX_HOST_PTRS)) return fail(ctx, HDGX_ERR_ARG, "flags");
@@ -593,7 +607,7 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   // per-element workspace for the Schur inputs
   const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
                      (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
-  const int64_t chunk = ws_chunk(ne, per * 8);
+  const int64_t chunk = ws_chunk(ne, per * 8, ctx->ws_cap);
   if (ne == 0) return HDGX_OK;
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
@@ -678,7 +692,7 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
   HIP_TRY(ctx, hipMemcpyAsync(dapps, apps.data(), sizeof(DevApp) * nseg, hipMemcpyHostToDevice, ctx->stream));
   const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
                      (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
-  const int64_t chunk = ws_chunk(total, per * 8);
+  const int64_t chunk = ws_chunk(total, per * 8, ctx->ws_cap);
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   for (int64_t g0 = 0; g0 < total; g0 += chunk) {
     const int64_t n = std::min(chunk, total - g0);
@@ -735,6 +749,25 @@ struct hdgx_plan {
 
 namespace {
 
+// frees everything a plan holds (a partly built one too)
+void plan_free(hdgx_plan* plan) {
+  if (!plan) return;
+  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
+  if (plan->cap) (void)hipStreamDestroy(plan->cap);
+  for (int p = 0; p <= HDGX_MAXP; ++p) {
+    if (plan->side[p]) (void)hipStreamDestroy(plan->side[p]);
+    if (plan->join[p]) (void)hipEventDestroy(plan->join[p]);
+  }
+  if (plan->fork) (void)hipEventDestroy(plan->fork);
+  if (plan->ws) (void)hipFree(plan->ws);
+  if (plan->dev) (void)hipFree(plan->dev);
+  delete plan;
+}
+
+struct PlanDeleter {
+  void operator()(hdgx_plan* p) const { plan_free(p); }
+};
+
 int plan_record(hdgx_ctx* ctx, hdgx_plan* P, hipStream_t s) {
   if (P->mode == HDGX_PLAN_FUSED) {
     return with_family(P->phys, P->nd, [&](auto ph, auto nd) {
@@ -774,10 +807,13 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   const hdgx_app* a0 = segs[0].app;
   if (!a0 || !masters[0]) return fail(ctx, HDGX_ERR_ARG, "plan: null app or master");
-  auto P = std::make_unique<hdgx_plan>();
+  // every error return below frees what the plan holds by then
+  std::unique_ptr<hdgx_plan, PlanDeleter> P(new hdgx_plan());
   P->mode = mode;
   P->phys = a0->physics;
   P->nd = masters[0]->nd;
+  if (mode == HDGX_PLAN_FUSED && with_family(P->phys, P->nd, [](auto, auto) { return 0; }) < 0)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no fused kernel for this physics and dimension");
   // order the segments by decreasing p (stable)
   std::vector<int> order(nseg), pof(nseg);
   std::vector<DevApp> apps(nseg);
@@ -904,9 +940,6 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
   int ncu_dev = 256;
   (void)hipDeviceGetAttribute(&ncu_dev, hipDeviceAttributeMultiprocessorCount, ctx->device);
   P->grid = grid > 0 ? grid : 4 * ncu_dev;
-  if (mode == HDGX_PLAN_FUSED &&
-      with_family(P->phys, P->nd, [](auto, auto) { return 0; }) < 0)
-    return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no fused kernel for this physics and dimension");
   if (mode == HDGX_PLAN_CONCURRENT) {
     HIP_TRY(ctx, hipEventCreateWithFlags(&P->fork, hipEventDisableTiming));
     for (int p = 1; p < HDGX_MAXP; ++p) {
@@ -955,16 +988,7 @@ int hdgx_plan_destroy(hdgx_ctx* ctx, hdgx_plan* plan) {
   if (!plan) return HDGX_OK;
   if (ctx) (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
-  if (plan->exec) (void)hipGraphExecDestroy(plan->exec);
-  if (plan->cap) (void)hipStreamDestroy(plan->cap);
-  for (int p = 0; p <= HDGX_MAXP; ++p) {
-    if (plan->side[p]) (void)hipStreamDestroy(plan->side[p]);
-    if (plan->join[p]) (void)hipEventDestroy(plan->join[p]);
-  }
-  if (plan->fork) (void)hipEventDestroy(plan->fork);
-  if (plan->ws) (void)hipFree(plan->ws);
-  if (plan->dev) (void)hipFree(plan->dev);
-  delete plan;
+  plan_free(plan);
   return HDGX_OK;
 }
 

@@ -26,6 +26,7 @@ struct hdgx_ctx {
   std::map<uint64_t, MasterBuf> masters;
   void* ws = nullptr;            // element matrices between the quadrature and Schur kernels
   size_t ws_bytes = 0;
+  size_t ws_cap = (size_t)24 << 30;  // most bytes ws may hold (hdgx_set_workspace_limit)
   void* tws = nullptr;           // trace-solve workspace (Krylov basis, reductions)
   size_t tws_bytes = 0;
   double* pinned = nullptr;      // small host staging for reductions

@@ -50,7 +50,10 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
   hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QUAD_THREADS), 0, s, t, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm);
+  if constexpr (SchurLdsShape<C>::USE)
+    hipLaunchKernelGGL(schur_seg_lds_kernel<C>, dim3((unsigned)t.total), dim3(SCHUR_LDS_THREADS), 0, s, t);
+  else
+    hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm);
   return hipGetLastError();
 }
 

@@ -175,8 +175,10 @@ __global__ void face_rows_kernel(int nd, int64_t ne, const int32_t* __restrict__
   }
 }
 
-// one thread per (face node b, face i): elcon of both sides
-__global__ void elcon_kernel(int ncmp, int npv, int ncd, int npf, int nfe, int64_t nf, int nvf,
+// one thread per (face node b, face i): elcon of both sides.  The element ids
+// of f and the node ids of perm are the caller's: any outside [0, ne) or
+// [0, npv) sets error bit 16 before it indexes anything.
+__global__ void elcon_kernel(int ncmp, int npv, int ncd, int npf, int nfe, int64_t ne, int64_t nf, int nvf,
                              const double* __restrict__ dgnodes, const int32_t* __restrict__ perm,
                              const int32_t* __restrict__ f, const int32_t* __restrict__ t2f, double tol, int32_t* elcon,
                              int* err) {
@@ -186,6 +188,9 @@ __global__ void elcon_kernel(int ncmp, int npv, int ncd, int npf, int nfe, int64
   const int64_t i = idx / npf;
   const int32_t* row = f + (int64_t)(nvf + 2) * i;
   const int32_t e1 = row[nvf], e2 = row[nvf + 1];
+  if (e1 < 0 || e1 >= ne || e2 >= ne) { atomicOr(err, 16); return; }
+  for (int k = 0; k < npf * nfe; ++k)
+    if (perm[k] < 0 || perm[k] >= npv) { atomicOr(err, 16); return; }
   int j1 = -1, j2 = -1;
   for (int j = 0; j < nfe; ++j) {
     if (t2f[j + (int64_t)nfe * e1] == i && j1 < 0) j1 = j;
@@ -216,15 +221,17 @@ __global__ void elcon_kernel(int ncmp, int npv, int ncd, int npf, int nfe, int64
 
 // one thread per face: fconn (4, nf) = [e1, j1, e2 or -1, j2 or 0] and the
 // face-to-face table f2f (2 nfe - 1, nf): the face, the other faces of its
-// first element, then of its second (ascending local index), -1 where none
-__global__ void f2f_kernel(int nd, int64_t nf, const int32_t* __restrict__ f, const int32_t* __restrict__ t2f,
-                           int32_t* fconn, int32_t* f2f, int* err) {
+// first element, then of its second (ascending local index), -1 where none.
+// Element ids of f outside [0, ne) set error bit 16 before any t2f read.
+__global__ void f2f_kernel(int nd, int64_t ne, int64_t nf, const int32_t* __restrict__ f,
+                           const int32_t* __restrict__ t2f, int32_t* fconn, int32_t* f2f, int* err) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nf) return;
   const int nvf = nd, nfe = nd + 1, nbf = 2 * nfe - 1;
   const int32_t* row = f + (int64_t)(nvf + 2) * i;
   const int32_t e1 = row[nvf];
   const int32_t e2 = row[nvf + 1] >= 0 ? row[nvf + 1] : -1;
+  if (e1 < 0 || e1 >= ne || e2 >= ne) { atomicOr(err, 16); return; }
   int j1 = -1, j2 = -1;
   for (int j = 0; j < nfe; ++j) {
     if (j1 < 0 && t2f[j + (int64_t)nfe * e1] == i) j1 = j;
@@ -328,21 +335,22 @@ int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
   const int ncmp = ncd < 3 ? ncd : 3;              // coordinates compared
-  hipLaunchKernelGGL(elcon_kernel, dim3(grid_of(nf * npf, 256)), dim3(256), 0, s, ncmp, npv, ncd, npf, nd + 1, nf, nd,
-                     dgnodes, perm, f, t2f, 1e-16, elcon, err);
+  hipLaunchKernelGGL(elcon_kernel, dim3(grid_of(nf * npf, 256)), dim3(256), 0, s, ncmp, npv, ncd, npf, nd + 1, ne, nf,
+                     nd, dgnodes, perm, f, t2f, 1e-16, elcon, err);
   HIP_TRY(ctx, hipGetLastError());
   int herr = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (herr & 16) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: an element id of f outside 0..ne-1 or a perm entry outside 0..npv-1");
   if (herr & 4) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: a face missing from its element's t2f");
   if (herr & 8) return fail(ctx, HDGX_ERR_ARG, "mesh_elcon: trace nodes do not match across a face");
   return HDGX_OK;
 }
 
-int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
+int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t ne, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
                   int32_t* f2f) {
   if (!ctx) return HDGX_ERR_ARG;
-  if ((nd != 2 && nd != 3) || nf < 0) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: sizes");
+  if ((nd != 2 && nd != 3) || ne < 0 || nf < 0) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: sizes");
   if (nf == 0) return HDGX_OK;
   if (!f || !t2f || !f2f) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: null array");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
@@ -351,11 +359,12 @@ int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int
   int* err = static_cast<int*>(ctx->mws);
   hipStream_t s = ctx->stream;
   HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int), s));
-  hipLaunchKernelGGL(f2f_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nd, nf, f, t2f, fconn, f2f, err);
+  hipLaunchKernelGGL(f2f_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nd, ne, nf, f, t2f, fconn, f2f, err);
   HIP_TRY(ctx, hipGetLastError());
   int herr = 0;
   HIP_TRY(ctx, hipMemcpyAsync(&herr, err, sizeof(int), hipMemcpyDeviceToHost, s));
   HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (herr & 16) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: an element id of f outside 0..ne-1");
   if (herr) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: a face missing from its element's t2f");
   return HDGX_OK;
 }

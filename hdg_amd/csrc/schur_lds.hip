@@ -42,8 +42,9 @@ HDGX_DEV void argmax_step(double& v, int& i, double v2, int i2) {
   if (v2 > v || (v2 == v && i2 < i)) { v = v2; i = i2; }
 }
 
+// Element e of the batch A, one SCHUR_LDS_THREADS-thread workgroup.
 template <class C>
-__global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_lds_kernel(SchurArgs A) {
+HDGX_DEV void schur_lds_body(const SchurArgs& A, const int64_t e) {
   using S = SchurLdsShape<C>;
   constexpr int NU = S::NU, NH = S::NH, LDA = S::LDA, NT = SCHUR_LDS_THREADS, GR = S::GR;
   __shared__ double sA[NU * LDA];
@@ -51,7 +52,6 @@ __global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_lds_kernel(SchurArgs 
   __shared__ int sPerm[NU];
   __shared__ int sPiv;
   __shared__ int sSt;
-  const int64_t e = blockIdx.x;
   if (e >= A.ne) return;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 
@@ -179,6 +179,11 @@ __global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_lds_kernel(SchurArgs 
   if (st) atomicOr(&sSt, st);
   __syncthreads();
   if (tid == 0 && A.status) A.status[e] = sSt;
+}
+
+template <class C>
+__global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_lds_kernel(SchurArgs A) {
+  schur_lds_body<C>(A, blockIdx.x);
 }
 
 }  // namespace hdgx

@@ -68,11 +68,11 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
   quad_body<C>(A, T, ap, 0, lds);
 }
 
+// the elimination arguments of batch element g (workspace slot w) as a
+// one-element SchurArgs
 template <class C>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm) {
+HDGX_DEV SchurArgs seg_schur_args(const SegTable& S, int64_t w) {
   constexpr int NPV = C::NPV, NU = C::NU, NH = C::NH, NCOL = C::NCOL, ND = C::ND, NDF = C::NDF, NCU = C::NCU;
-  const int64_t w = blockIdx.x;
-  if (w >= S.total) return;
   const int64_t g = S.g0 + w;
   const ElemSeg sg = uload_obj(S.segs + find_seg(S.segs, S.nseg, g));
   const int64_t e = g - sg.e0;
@@ -99,8 +99,27 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>
   A.status = sg.status ? sg.status + e : nullptr;
   A.prec = 0;
   A.nco = C::NC;
+  return A;
+}
+
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm) {
+  const int64_t w = blockIdx.x;
+  if (w >= S.total) return;
+  const SchurArgs A = seg_schur_args<C>(S, w);
   __shared__ SchurLds<C> lds;
   schur_mfma_body<C>(A, perm, 0, lds);
+}
+
+// the configurations hdgx_elemental eliminates in LDS (SchurLdsShape<C>::USE)
+// take the same body here, so that a batch reproduces each request's own
+// call bit for bit
+template <class C>
+__global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_seg_lds_kernel(SegTable S) {
+  const int64_t w = blockIdx.x;
+  if (w >= S.total) return;
+  const SchurArgs A = seg_schur_args<C>(S, w);
+  schur_lds_body<C>(A, 0);
 }
 
 }  // namespace hdgx

@@ -43,6 +43,7 @@ def lib():
         L.hdgx_set_stream.argtypes = [vp, vp]
         L.hdgx_set_precision.argtypes = [vp, i32]
         L.hdgx_set_flags.argtypes = [vp, i32]
+        L.hdgx_set_workspace_limit.argtypes = [vp, i64]
         L.hdgx_last_error.argtypes = [vp]
         L.hdgx_last_error.restype = ctypes.c_char_p
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
@@ -69,12 +70,13 @@ def lib():
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
         L.hdgx_mesh_faces.argtypes = [vp, i32, i64, i32, vp, vp, vp, vp]
         L.hdgx_mesh_elcon.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, vp, vp]
-        L.hdgx_mesh_f2f.argtypes = [vp, i32, i64, vp, vp, vp, vp]
+        L.hdgx_mesh_f2f.argtypes = [vp, i32, i64, i64, vp, vp, vp, vp]
         _lib = L
     return _lib
 
 
-EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags", "hdgx_last_error",
+EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags",
+           "hdgx_set_workspace_limit", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
            "hdgx_elemental_batch",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
@@ -171,6 +173,11 @@ class Context:
         """hdgx_set_flags: INPLACE makes schur_u / schur_uq overwrite their
         (device) inputs as the MEX files do."""
         self._check(lib().hdgx_set_flags(self.h, int(flags)))
+
+    def set_workspace_limit(self, nbytes):
+        """hdgx_set_workspace_limit: most bytes of the element-matrix
+        workspace (0 = the 24 GiB default); larger calls run in chunks."""
+        self._check(lib().hdgx_set_workspace_limit(self.h, int(nbytes)))
 
     def _check(self, rc):
         if rc:
@@ -335,9 +342,10 @@ class Context:
         f2f (2 nfe - 1, nf)) Dev arrays, equal to hdg_amd.trace's
         face_connectivity and mkf2f."""
         nd = f.shape[0] - 2
+        ne = t2f.shape[1]
         fconn = Dev.empty((4, nf), "int32")
         f2f = Dev.empty((2 * nd + 1, nf), "int32")
-        self._check(lib().hdgx_mesh_f2f(self.h, nd, nf, f.ptr, t2f.ptr, fconn.ptr, f2f.ptr))
+        self._check(lib().hdgx_mesh_f2f(self.h, nd, ne, nf, f.ptr, t2f.ptr, fconn.ptr, f2f.ptr))
         return fconn, f2f
 
     def getq(self, master, dgnodes, udg, uh, sh=None, fc_q=1.0, ncu=None, out=None, inplace=False, shapnv=None):

@@ -120,6 +120,15 @@ int hdgx_set_precision(hdgx_ctx* ctx, int precision);
  *                 asynchronous.  The trace-solve entry points always take
  *                 device memory. */
 int hdgx_set_flags(hdgx_ctx* ctx, int flags);
+/* Most device bytes of the context's element-matrix workspace, which carries
+ * the element matrices from the quadrature to the elimination inside
+ * hdgx_elemental / hdgx_elemental_batch (0 = the default, 24 GiB).  A call
+ * whose elements need more runs in chunks of nearly equal size that fit it,
+ * as the reference blocks its element loop (kernel/hdg_elemental.m:9-13);
+ * results do not depend on the chunking.  The workspace is allocated at the
+ * first call that needs it, grows up to the limit, and is held until
+ * hdgx_destroy (or freed here when it is above a new, lower limit). */
+int hdgx_set_workspace_limit(hdgx_ctx* ctx, int64_t bytes);
 const char* hdgx_last_error(hdgx_ctx* ctx);
 
 /* Elemental operator.  Replaces
@@ -219,7 +228,9 @@ typedef struct hdgx_segment {
  * request (kernel/hdg_elemental.m:1-203, as called per mesh partition by
  * kernel/hdg_assemble.m:40-43), with its own physics parameters.  All segments
  * share the master (nd, p) and the app's physics, nd, ncu, nc and localsolve;
- * results equal those of hdgx_elemental on each segment alone. */
+ * results equal those of hdgx_elemental on each segment alone (bit for bit in
+ * 2D; for tetrahedra to rounding, 1e-13 relative, as the batched quadrature
+ * kernel contracts a few multiply-adds differently). */
 int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg,
                          const hdgx_segment* segs);
 
@@ -231,7 +242,10 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg,
  *   HDGX_PLAN_CONCURRENT  one batched launch per order, on concurrent streams;
  *   HDGX_PLAN_FUSED       one persistent launch per kernel over all orders
  *                         (grid workgroups; 0 = 4 per CU).
- * Results are those of hdgx_elemental on each segment (FP64).  The plan
+ * Results are those of hdgx_elemental on each segment (FP64): bit for bit in
+ * the per-order and concurrent modes; the fused mode eliminates every order
+ * in the FP64 MFMA form, so where hdgx_elemental takes the LDS form
+ * (tetrahedra p = 2) they agree to rounding (1e-12 relative).  The plan
  * owns its workspace; it has no counterpart in the reference, whose element
  * loop runs one mesh at a time (kernel/hdg_assemble.m:40-43). */
 enum { HDGX_PLAN_PER_ORDER = 0, HDGX_PLAN_FUSED = 1, HDGX_PLAN_CONCURRENT = 2 };
@@ -339,7 +353,8 @@ int hdgx_mesh_faces(hdgx_ctx* ctx, int nd, int64_t ne, int32_t nvert, const int3
  * past 1e-16).
  *   dgnodes (npv, ncd, ne) FP64; perm (npf, nd+1) int32 (master.perm,
  *   0-based); f, t2f as hdgx_mesh_faces gives them; elcon (npf*(nd+1), ne)
- *   int32 out.  Device arrays; synchronises the context's stream. */
+ *   int32 out.  Device arrays; synchronises the context's stream.  An element
+ *   id of f outside 0..ne-1 or a perm entry outside 0..npv-1 is HDGX_ERR_ARG. */
 int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne, int64_t nf, const double* dgnodes,
                     const int32_t* perm, const int32_t* f, const int32_t* t2f, int32_t* elcon);
 
@@ -348,9 +363,10 @@ int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne
  * then of its second, in ascending local index, -1 where there is no second
  * element; fconn (4, nf) int32 out (may be NULL): first element, its local
  * face, second element or -1, its local face or 0 (hdgx_trace.fconn).
- * f, t2f as hdgx_mesh_faces gives them (a negative second element, e.g. a
- * boundary tag, means none).  Device arrays; synchronises the stream. */
-int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
+ * f, t2f as hdgx_mesh_faces gives them for ne elements (a negative second
+ * element, e.g. a boundary tag, means none; an element id of f at or past ne
+ * is HDGX_ERR_ARG).  Device arrays; synchronises the stream. */
+int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t ne, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
                   int32_t* f2f);
 
 /* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace

@@ -18,6 +18,8 @@ from hdg_amd.master import mkmaster  # noqa: E402
 from hdg_amd.mesh import _orient, createnodes, cubemesh, hdgelcon, jitter, mkt2f, squaremesh  # noqa: E402
 from hdg_amd.runtime import HdgxError  # noqa: E402
 from hdg_amd.trace import face_connectivity, mkf2f  # noqa: E402
+from hdg_amd.runtime import Dev  # noqa: E402
+from tests.test_mkt2f_bruteforce import MESHES, mkt2f_loop  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -68,6 +70,45 @@ def test_one_and_no_element(ctx):
     _check(ctx, np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]]), np.array([[0, 1, 2, 3]]), mkmaster(3, 2))
     f, t2f, nf = ctx.mesh_faces(np.zeros((0, 3), dtype=np.int64), nvert=0)
     assert nf == 0
+
+
+@pytest.mark.parametrize("name", list(MESHES))
+def test_faces_match_reference_loop(ctx, name):
+    """hdgx_mesh_faces against the literal restatement of mkt2f.m:58-80's
+    first-visit loop (not against hdg_amd/mesh.py), integer for integer."""
+    t = MESHES[name]()
+    print(f"[gpu] device mesh vs mkt2f loop: {name} ne={t.shape[0]}", flush=True)
+    f_ref, t2f_ref = mkt2f_loop(t, t.shape[1] - 1)
+    f, t2f, nf = ctx.mesh_faces(t)
+    assert nf == f_ref.shape[0]
+    assert np.array_equal(f.numpy().T, f_ref)
+    assert np.array_equal(t2f.numpy().T, t2f_ref)
+
+
+def test_malformed_face_table(ctx):
+    """An f whose element ids are 1-based (the reference's own numbering) or
+    otherwise past ne is refused before any kernel indexes with it."""
+    print("[gpu] device mesh: malformed f", flush=True)
+    m = mkmaster(2, 2)
+    p, t = squaremesh(4, 4)
+    f, t2f, nf = ctx.mesh_faces(t)
+    dg = createnodes(p, t, m.plocvl)
+    bad = f.numpy().copy()
+    bad[2:, :] += 1                              # element ids 1-based: the last element becomes ne
+    fb = Dev.from_numpy(bad, np.int32)
+    with pytest.raises(HdgxError, match="outside 0..ne-1"):
+        ctx.mesh_elcon(dg, m.perm, fb, t2f, nf)
+    with pytest.raises(HdgxError, match="outside 0..ne-1"):
+        ctx.mesh_f2f(fb, t2f, nf)
+    # a perm entry past npv
+    perm = m.perm.copy()
+    perm[0, 0] = m.npv
+    with pytest.raises(HdgxError, match="perm entry"):
+        ctx.mesh_elcon(dg, perm, f, t2f, nf)
+    # the good tables still work afterwards
+    el = ctx.mesh_elcon(dg, m.perm, f, t2f, nf)
+    el_ref, _ = hdgelcon(dg, m.perm, *mkt2f(t, 2))
+    assert np.array_equal(el.numpy(), el_ref)
 
 
 def test_errors(ctx):

@@ -101,6 +101,94 @@ def test_ns2d_flux_inviscid_limit(oracle):
     assert np.abs(fns - feu).max() < 1e-13
 
 
+def _ns_textbook(gam, re, pr, prim, grad):
+    """Compressible Navier-Stokes flux in the nondimensional form of
+    application/FM/ns/flux2d.m, written from primitive fields: the Euler flux
+    minus the viscous flux of a Newtonian fluid (Stokes' hypothesis) with
+    Fourier heat conduction in T = p/rho,
+      tau = (1/Re) (grad v + grad v^T - 2/3 div v I),
+      q_heat = gam / ((gam-1) Re Pr) grad(p/rho).
+    prim (n, 4) = rho, u, v, p; grad (n, 4, 2) their x, y derivatives."""
+    r, u, v, p = prim.T
+    (rx, ry), (ux, uy), (vx, vy), (px, py) = (grad[:, i, :].T for i in range(4))
+    rE = p / (gam - 1) + 0.5 * r * (u * u + v * v)
+    mu = 1.0 / re
+    txx = mu * (2 * ux - (2.0 / 3.0) * (ux + vy))
+    tyy = mu * (2 * vy - (2.0 / 3.0) * (ux + vy))
+    txy = mu * (uy + vx)
+    kappa = gam / ((gam - 1) * re * pr)
+    Tx = (px * r - p * rx) / r ** 2
+    Ty = (py * r - p * ry) / r ** 2
+    f = np.zeros((prim.shape[0], 4, 2))
+    f[:, :, 0] = np.stack([r * u, r * u * u + p - txx, r * u * v - txy, u * (rE + p) - (u * txx + v * txy + kappa * Tx)], 1)
+    f[:, :, 1] = np.stack([r * v, r * u * v - txy, r * v * v + p - tyy, v * (rE + p) - (u * txy + v * tyy + kappa * Ty)], 1)
+    return f
+
+
+def _ns_udg_from_primitive(gam, prim, grad):
+    """udg = (U, q) with U the conservative state and q = -grad U (the HDG
+    convention q + grad u = 0, kernel/volint.m:57-60), by the product rule."""
+    r, u, v, p = prim.T
+    U = np.stack([r, r * u, r * v, p / (gam - 1) + 0.5 * r * (u * u + v * v)], 1)
+    q = np.zeros((prim.shape[0], 4, 2))
+    for d in range(2):
+        rd, ud, vd, pd = (grad[:, i, d] for i in range(4))
+        q[:, 0, d] = -rd
+        q[:, 1, d] = -(u * rd + r * ud)
+        q[:, 2, d] = -(v * rd + r * vd)
+        q[:, 3, d] = -(pd / (gam - 1) + 0.5 * (u * u + v * v) * rd + r * (u * ud + v * vd))
+    return np.hstack([U, q.reshape(-1, 8, order="F")])
+
+
+def test_ns2d_flux_closed_form(oracle):
+    """The oracle's viscous stress and heat flux against the textbook
+    Navier-Stokes flux (application/FM/ns/flux2d.m:107 generates the same
+    expressions symbolically).  Re = 10 makes the viscous part a tenth of the
+    flux, so a swapped 2/3 <-> 4/3 in tau, a missing gam in the conduction
+    coefficient or a sign slip in tau_xy changes f by far more than the bar."""
+    rng = np.random.default_rng(17)
+    n, gam, re, pr = 24, 1.4, 10.0, 0.72
+    prim = np.stack([1.0 + 0.3 * rng.random(n), 0.5 + 0.5 * rng.random(n),
+                     0.4 * rng.standard_normal(n), 2.0 + rng.random(n)], 1)
+    grad = rng.standard_normal((n, 4, 2))
+    udg = _ns_udg_from_primitive(gam, prim, grad)
+    f, _ = oracle.flux(ns_app(gam=gam, re=re, pr=pr), rng.random((n, 2)), udg)
+    ref = _ns_textbook(gam, re, pr, prim, grad)
+    assert np.abs(f - ref).max() < 1e-12 * np.abs(ref).max()
+    # the viscous part is large enough for the planted mistakes to show
+    inv = _ns_textbook(gam, 1e300, pr, prim, grad)
+    assert np.abs(ref - inv).max() > 1e-3 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("mistake", ["c23", "gamma", "txy_sign"])
+def test_ns2d_flux_closed_form_catches_mistakes(oracle, mistake):
+    """The closed-form check fails for each plausible slip in the viscous
+    terms: planted in the textbook side (on the CPU), never in the product."""
+    rng = np.random.default_rng(18)
+    n, gam, re, pr = 24, 1.4, 10.0, 0.72
+    prim = np.stack([1.0 + 0.3 * rng.random(n), 0.5 + 0.5 * rng.random(n),
+                     0.4 * rng.standard_normal(n), 2.0 + rng.random(n)], 1)
+    grad = rng.standard_normal((n, 4, 2))
+    udg = _ns_udg_from_primitive(gam, prim, grad)
+    f, _ = oracle.flux(ns_app(gam=gam, re=re, pr=pr), rng.random((n, 2)), udg)
+    ref = _ns_textbook(gam, re, pr, prim, grad)
+    r, u, v, p = prim.T
+    (rx, ry), (ux, uy), (vx, vy), (px, py) = (grad[:, i, :].T for i in range(4))
+    bad = ref.copy()
+    if mistake == "c23":      # tau_xx with 2/3 and 4/3 swapped
+        d = (1.0 / re) * ((2.0 / 3.0 * ux - 4.0 / 3.0 * vy) - (4.0 / 3.0 * ux - 2.0 / 3.0 * vy))
+        bad[:, 1, 0] -= d
+        bad[:, 3, 0] -= u * d
+    elif mistake == "gamma":  # conduction coefficient without gam
+        Tx = (px * r - p * rx) / r ** 2
+        bad[:, 3, 0] += (gam - 1.0) / ((gam - 1) * re * pr) * Tx
+    else:                     # tau_xy with the wrong sign
+        txy = (1.0 / re) * (uy + vx)
+        bad[:, 2, 0] += 2 * txy
+        bad[:, 1, 1] += 2 * txy
+    assert np.abs(f - bad).max() > 1e-6 * np.abs(ref).max()
+
+
 @pytest.mark.parametrize("which", ["euler2d", "ns2d", "euler3d"])
 def test_fhat_jacobians_fd(oracle, which):
     rng = np.random.default_rng(4)
