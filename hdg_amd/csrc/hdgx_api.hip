@@ -44,9 +44,16 @@ uint64_t fnv(uint64_t h, const void* p, size_t n) {
   return h;
 }
 
-// Upload (once per distinct table content) the master tables plus the
-// node-on-face lookup loc(v, j) derived from perm.
-int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
+// The master tables packed for the device: one host image of every table plus
+// the node-on-face lookup loc(v, j) derived from perm, the byte offset of each
+// table in it, and a content hash (FNV-1a over the sizes and every table).
+struct MasterImage {
+  std::vector<char> host;
+  size_t off[11];
+  uint64_t hash;
+};
+
+int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   if (!m || !m->shapvt || !m->shapvg || !m->shapvgdotshapvl || !m->shapft || !m->shapfg || !m->shapfgdotshapfc ||
       !m->shapmv || !m->shapmf || !m->perm || !m->permgeom)
     return fail(ctx, HDGX_ERR_ARG, "master: null table");
@@ -64,9 +71,7 @@ int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
   h = fnv(h, m->shapft, n_ft * 8); h = fnv(h, m->shapfg, n_fg * 8); h = fnv(h, m->shapfgdotshapfc, n_ff * 8);
   h = fnv(h, m->shapmv, n_mv * 8); h = fnv(h, m->shapmf, n_mf * 8);
   h = fnv(h, m->perm, n_perm * 4); h = fnv(h, m->permgeom, n_perm * 4);
-  auto it = ctx->masters.find(h);
-  if (it != ctx->masters.end()) { *out = &it->second; return HDGX_OK; }
-
+  img->hash = h;
   std::vector<int32_t> loc(n_loc, -1);
   for (int j = 0; j < nfe; ++j)
     for (int a = 0; a < npf; ++a) {
@@ -77,29 +82,40 @@ int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
   for (size_t i = 0; i < n_perm; ++i)
     if (m->permgeom[i] < 0 || m->permgeom[i] >= npv) return fail(ctx, HDGX_ERR_ARG, "master: permgeom out of range");
   const size_t nd64 = n_vt + n_vg + n_vv + n_ft + n_fg + n_ff + n_mv + n_mf;
-  const size_t bytes = nd64 * 8 + (2 * n_perm + n_loc) * 4;
-  std::vector<char> host(bytes);
-  char* p = host.data();
-  auto put = [&](const void* src, size_t n) { std::memcpy(p, src, n); char* r = p; p += n; return r; };
-  std::vector<char*> offs;
-  offs.push_back(put(m->shapvt, n_vt * 8)); offs.push_back(put(m->shapvg, n_vg * 8));
-  offs.push_back(put(m->shapvgdotshapvl, n_vv * 8)); offs.push_back(put(m->shapft, n_ft * 8));
-  offs.push_back(put(m->shapfg, n_fg * 8)); offs.push_back(put(m->shapfgdotshapfc, n_ff * 8));
-  offs.push_back(put(m->shapmv, n_mv * 8)); offs.push_back(put(m->shapmf, n_mf * 8));
-  offs.push_back(put(m->perm, n_perm * 4)); offs.push_back(put(m->permgeom, n_perm * 4));
-  offs.push_back(put(loc.data(), n_loc * 4));
+  img->host.assign(nd64 * 8 + (2 * n_perm + n_loc) * 4, 0);
+  size_t at = 0;
+  int k = 0;
+  auto put = [&](const void* src, size_t n) { std::memcpy(img->host.data() + at, src, n); img->off[k++] = at; at += n; };
+  put(m->shapvt, n_vt * 8); put(m->shapvg, n_vg * 8); put(m->shapvgdotshapvl, n_vv * 8); put(m->shapft, n_ft * 8);
+  put(m->shapfg, n_fg * 8); put(m->shapfgdotshapfc, n_ff * 8); put(m->shapmv, n_mv * 8); put(m->shapmf, n_mf * 8);
+  put(m->perm, n_perm * 4); put(m->permgeom, n_perm * 4); put(loc.data(), n_loc * 4);
+  return HDGX_OK;
+}
+
+// the device view of a master image copied to `base`
+void bind_master(const hdgx_master* m, const MasterImage& img, char* base, MasterBuf* mb) {
+  auto dp = [&](int i) { return base + img.off[i]; };
+  mb->tab.shapvt = (const double*)dp(0); mb->tab.shapvg = (const double*)dp(1); mb->tab.svv = (const double*)dp(2);
+  mb->tab.shapft = (const double*)dp(3); mb->tab.shapfg = (const double*)dp(4); mb->tab.sff = (const double*)dp(5);
+  mb->tab.shapmv = (const double*)dp(6); mb->tab.shapmf = (const double*)dp(7);
+  mb->tab.perm = (const int32_t*)dp(8); mb->tab.permgeom = (const int32_t*)dp(9); mb->tab.loc = (const int32_t*)dp(10);
+  mb->nd = m->nd; mb->porder = m->porder; mb->npv = m->npv; mb->npf = m->npf; mb->nfe = m->nfe;
+  mb->ngv = m->ngv; mb->ngf = m->ngf;
+}
+
+// Upload (once per distinct table content and context) the master tables.
+int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
+  MasterImage img;
+  int rc = pack_master(ctx, m, &img);
+  if (rc) return rc;
+  auto it = ctx->masters.find(img.hash);
+  if (it != ctx->masters.end()) { *out = &it->second; return HDGX_OK; }
   MasterBuf mb;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
-  HIP_TRY(ctx, hipMalloc(&mb.dev, bytes));
-  HIP_TRY(ctx, hipMemcpy(mb.dev, host.data(), bytes, hipMemcpyHostToDevice));
-  char* base = static_cast<char*>(mb.dev);
-  auto dp = [&](int i) { return base + (offs[i] - host.data()); };
-  mb.tab.shapvt = (const double*)dp(0); mb.tab.shapvg = (const double*)dp(1); mb.tab.svv = (const double*)dp(2);
-  mb.tab.shapft = (const double*)dp(3); mb.tab.shapfg = (const double*)dp(4); mb.tab.sff = (const double*)dp(5);
-  mb.tab.shapmv = (const double*)dp(6); mb.tab.shapmf = (const double*)dp(7);
-  mb.tab.perm = (const int32_t*)dp(8); mb.tab.permgeom = (const int32_t*)dp(9); mb.tab.loc = (const int32_t*)dp(10);
-  mb.nd = nd; mb.porder = m->porder; mb.npv = npv; mb.npf = npf; mb.nfe = nfe; mb.ngv = ngv; mb.ngf = ngf;
-  auto res = ctx->masters.emplace(h, mb);
+  HIP_TRY(ctx, hipMalloc(&mb.dev, img.host.size()));
+  HIP_TRY(ctx, hipMemcpy(mb.dev, img.host.data(), img.host.size(), hipMemcpyHostToDevice));
+  bind_master(m, img, static_cast<char*>(mb.dev), &mb);
+  auto res = ctx->masters.emplace(img.hash, mb);
   *out = &res.first->second;
   return HDGX_OK;
 }
@@ -221,6 +237,53 @@ int ensure_ws(hdgx_ctx* ctx, size_t bytes) {
   return HDGX_OK;
 }
 
+// hdgx_elemental's element loop over chunks of `chunk` elements, the element
+// matrices of each chunk in the workspace at ws (per_element_words * chunk
+// doubles): the quadrature kernel writes them, the elimination reads them.
+int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const hdgx_master* master,
+                       const hdgx_app* app, int64_t ne, const double* dgnodes, const int32_t* bf, const double* udg,
+                       const double* uh, const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
+                       int32_t* status, double* ws, int64_t chunk) {
+  const int nco = app->getdqdg == 1 ? app->nc : (app->getdqdg == 0 ? app->ncu : 0);
+  const int nd = master->nd, npv = master->npv, ncu = app->ncu, nc = app->nc;
+  const size_t ldo = (size_t)npv * nco;
+  const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
+  const int p = porder_of(nd, npv);
+  const bool uq = app->localsolve != 0;
+  for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
+    const int64_t n = std::min(chunk, ne - e0);
+    double* w = ws;
+    double* BD = w; w += (size_t)nu * ncol * n;
+    double* F = w; w += (size_t)nu * nh * n;
+    double* GK = w; w += (size_t)nh * ncol * n;
+    double* H = w; w += (size_t)nh * nh * n;
+    double* Ru = w; w += (size_t)nu * n;
+    double* Rh = w; w += (size_t)nh * n;
+    double *M = nullptr, *C = nullptr, *E = nullptr, *Rq = nullptr;
+    if (uq) {
+      M = w; w += (size_t)npv * npv * n;
+      C = w; w += (size_t)npv * npv * nd * n;
+      E = w; w += (size_t)npv * ndf * nd * n;
+      Rq = w; w += (size_t)npv * ncu * nd * n;
+    }
+    QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe, udg + (size_t)e0 * npv * nc,
+                uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr, BD, F, GK, H, Ru, Rh, M, C, E, Rq,
+                nullptr};
+    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
+                 nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
+                 status ? status + e0 : nullptr, ctx->prec, nco};
+    const int rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
+      using Cc = decltype(cfg);
+      int r = launch_quad<Cc>(ctx, mb, ap, qa);
+      if (r) return r;
+      return launch_schur<Cc>(ctx, sa, mb.tab.perm);
+    });
+    if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+    if (rc) return rc;
+  }
+  return HDGX_OK;
+}
+
 // HDGX_HOST_PTRS: an entry point's array arguments are host memory.  The
 // arrays are given device copies in the context's staging buffer (inputs
 // copied in, outputs copied back after the call, which then synchronises),
@@ -288,6 +351,48 @@ int without_host_ptrs(hdgx_ctx* ctx, F&& call) {
   return rc;
 }
 
+// ------------------------------------------- the allocation-free entry point
+// hdgx_elemental_noalloc keeps its master tables and its element-matrix
+// workspace in this file-scope device array instead of memory it allocates:
+// the first NOALLOC_TABLE_BYTES hold the packed master tables, the rest the
+// chunked workspace.  Every GPU the library runs on holds one copy for the
+// life of the process; all calls on one GPU share it.
+constexpr size_t NOALLOC_BYTES = (size_t)128 << 20;
+constexpr size_t NOALLOC_TABLE_BYTES = (size_t)1 << 20;
+static_assert(NOALLOC_TABLE_BYTES % 256 == 0, "the workspace part stays 256-byte aligned");
+
+// Host code must not pass this name to a kernel or a copy: on the host it is
+// a stand-in, not a device address.  Its address on each device is taken once
+// with hipGetSymbolAddress (noalloc_setup) and only that is used.
+static __device__ __align__(256) char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAXDEV = 64;
+struct NoallocDevice {
+  std::mutex mu;
+  int state = 0;             // 0: address not taken yet, 1: ptr valid, -1: hipGetSymbolAddress failed
+  char* ptr = nullptr;       // scratch's device address on this device
+  uint64_t table_hash = 0;   // content hash of the master tables now in the table area (0: none)
+  MasterBuf mb;              // their device view
+};
+NoallocDevice g_noalloc[NOALLOC_MAXDEV];
+
+// once per device, with that device current (from hdgx_create)
+void noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAXDEV) return;
+  NoallocDevice& d = g_noalloc[device];
+  std::lock_guard<std::mutex> lock(d.mu);
+  if (d.state != 0) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e == hipSuccess && p) {
+    d.ptr = static_cast<char*>(p);
+    d.state = 1;
+  } else {
+    (void)hipGetLastError();
+    d.state = -1;
+  }
+}
+
 }  // namespace
 
 extern "C" {
@@ -296,6 +401,8 @@ int hdgx_create(hdgx_ctx** ctx, int device) {
   if (!ctx) return HDGX_ERR_ARG;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return HDGX_ERR_DEVICE;
+  if (hipSetDevice(device) != hipSuccess) return HDGX_ERR_DEVICE;
+  noalloc_setup(device);
   auto* c = new hdgx_ctx();
   c->device = device;
   *ctx = c;
@@ -599,49 +706,58 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   DevApp ap;
   if ((rc = make_app(ctx, master, app, &ap))) return rc;
   if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
-  const int nd = master->nd, npv = master->npv, ncu = app->ncu, nc = app->nc;
-  const size_t ldo = (size_t)npv * nco;
-  const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
-  const int p = porder_of(nd, npv);
-  const bool uq = app->localsolve != 0;
-  // per-element workspace for the Schur inputs
-  const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
-                     (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
-  const int64_t chunk = ws_chunk(ne, per * 8, ctx->ws_cap);
   if (ne == 0) return HDGX_OK;
+  // per-element workspace for the Schur inputs
+  const size_t per = per_element_words(master->nd, master->npv, master->npf, master->nfe, app->ncu, app->nc,
+                                       app->localsolve != 0);
+  const int64_t chunk = ws_chunk(ne, per * 8, ctx->ws_cap);
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
-  for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
-    const int64_t n = std::min(chunk, ne - e0);
-    double* w = static_cast<double*>(ctx->ws);
-    double* BD = w; w += (size_t)nu * ncol * n;
-    double* F = w; w += (size_t)nu * nh * n;
-    double* GK = w; w += (size_t)nh * ncol * n;
-    double* H = w; w += (size_t)nh * nh * n;
-    double* Ru = w; w += (size_t)nu * n;
-    double* Rh = w; w += (size_t)nh * n;
-    double *M = nullptr, *C = nullptr, *E = nullptr, *Rq = nullptr;
-    if (uq) {
-      M = w; w += (size_t)npv * npv * n;
-      C = w; w += (size_t)npv * npv * nd * n;
-      E = w; w += (size_t)npv * ndf * nd * n;
-      Rq = w; w += (size_t)npv * ncu * nd * n;
-    }
-    QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe, udg + (size_t)e0 * npv * nc,
-                uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr, BD, F, GK, H, Ru, Rh, M, C, E, Rq,
-                nullptr};
-    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
-                 nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
-                 status ? status + e0 : nullptr, ctx->prec, nco};
-    rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
-      using Cc = decltype(cfg);
-      int r = launch_quad<Cc>(ctx, *mb, ap, qa);
-      if (r) return r;
-      return launch_schur<Cc>(ctx, sa, mb->tab.perm);
-    });
-    if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
-    if (rc) return rc;
+  return elemental_launches(ctx, *mb, ap, master, app, ne, dgnodes, bf, udg, uh, sh, ae, fe, dudg, dudg_duh, status,
+                            static_cast<double*>(ctx->ws), chunk);
+}
+
+int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                           const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
+                           const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
+                           int32_t* status) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+  if (ctx->flags & HDGX_HOST_PTRS)
+    return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: device arrays only (HDGX_HOST_PTRS stages through allocated memory)");
+  const int nco = app->getdqdg == 1 ? app->nc : (app->getdqdg == 0 ? app->ncu : 0);
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !ae || !fe || (nco > 0 && (!dudg || !dudg_duh))))
+    return fail(ctx, HDGX_ERR_ARG, "null array");
+  if (ctx->device < 0 || ctx->device >= NOALLOC_MAXDEV)
+    return fail(ctx, HDGX_ERR_UNSUPPORTED, "elemental_noalloc: device ordinal past the scratch table");
+  NoallocDevice& d = g_noalloc[ctx->device];
+  std::lock_guard<std::mutex> lock(d.mu);
+  if (d.state != 1) return fail(ctx, HDGX_ERR_DEVICE, "elemental_noalloc: hipGetSymbolAddress of the scratch array failed");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterImage img;
+  int rc = pack_master(ctx, master, &img);
+  if (rc) return rc;
+  if (img.host.size() > NOALLOC_TABLE_BYTES)
+    return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: master tables exceed the 1 MiB table area");
+  DevApp ap;
+  if ((rc = make_app(ctx, master, app, &ap))) return rc;
+  if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
+  const size_t per = 8 * per_element_words(master->nd, master->npv, master->npf, master->nfe, app->ncu, app->nc,
+                                           app->localsolve != 0);
+  const size_t ws_bytes = NOALLOC_BYTES - NOALLOC_TABLE_BYTES;
+  if (per > ws_bytes) return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: one element's matrices exceed the scratch");
+  if (ne == 0) return HDGX_OK;
+  if (d.table_hash != img.hash) {
+    // stream-ordered after earlier calls on this stream; the host image is a
+    // local, so the copy completes before the call goes on
+    HIP_TRY(ctx, hipMemcpyAsync(d.ptr, img.host.data(), img.host.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    bind_master(master, img, d.ptr, &d.mb);
+    d.table_hash = img.hash;
   }
-  return HDGX_OK;
+  const int64_t chunk = ws_chunk(ne, per, ws_bytes);
+  return elemental_launches(ctx, d.mb, ap, master, app, ne, dgnodes, bf, udg, uh, sh, ae, fe, dudg, dudg_duh, status,
+                            reinterpret_cast<double*>(d.ptr + NOALLOC_TABLE_BYTES), chunk);
 }
 
 int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, const hdgx_segment* segs) {

@@ -47,6 +47,7 @@ def lib():
         L.hdgx_last_error.argtypes = [vp]
         L.hdgx_last_error.restype = ctypes.c_char_p
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
+        L.hdgx_elemental_noalloc.argtypes = [vp, vp, vp, i64] + [vp] * 10
         L.hdgx_elemental_matrices.argtypes = [vp, vp, vp, i64] + [vp] * 15
         L.hdgx_residual.argtypes = [vp, vp, vp, i64] + [vp] * 8
         L.hdgx_getq.argtypes = [vp, vp, i32, i32, i32, i64] + [vp] * 4 + [ctypes.c_double, vp, i64, vp, vp]
@@ -78,7 +79,7 @@ def lib():
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags",
            "hdgx_set_workspace_limit", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
-           "hdgx_elemental_batch",
+           "hdgx_elemental_batch", "hdgx_elemental_noalloc",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2",
@@ -219,6 +220,19 @@ class Context:
         self._check(lib().hdgx_elemental(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
                                          h.ptr, _p(s), ae.ptr, fe.ptr, _p(dudg), _p(dudg_duh), st.ptr))
         return ae, fe, dudg, dudg_duh, st
+
+    def elemental_noalloc(self, master, app, dgnodes, bf, udg, uh, sh, out):
+        """hdgx_elemental_noalloc: elemental() with every array a device array
+        the caller already holds (`out` included); the library allocates
+        nothing (its tables and workspace live in a fixed device array)."""
+        ca = app.to_c() if isinstance(app, App) else app
+        mc = self.master_c(master)
+        dg, b, u, h, s = dgnodes, bf, udg, uh, sh
+        ae, fe, dudg, dudg_duh, st = out
+        self._check(lib().hdgx_elemental_noalloc(self.h, ctypes.byref(mc.c), ctypes.byref(ca), dg.shape[2], dg.ptr,
+                                                 b.ptr, u.ptr, h.ptr, _p(s), ae.ptr, fe.ptr, _p(dudg), _p(dudg_duh),
+                                                 _p(st)))
+        return out
 
     def _segments(self, segments):
         keep = []

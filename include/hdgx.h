@@ -145,6 +145,25 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
                    const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
                    int32_t* status);
 
+/* Elemental operator that allocates nothing.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Arguments, layouts, results and kernels are hdgx_elemental's
+ * (bit for bit); every array is device memory (HDGX_HOST_PTRS is refused).
+ * The master tables and the element matrices between the two kernels live in
+ * a 128 MiB device array of the library (1 MiB of tables, the rest
+ * workspace), which every GPU the library runs on holds until the process
+ * ends; the elements run in chunks that fit it, so ne is not limited, but a
+ * master whose tables pass 1 MiB is refused (HDGX_ERR_ARG).  The call makes
+ * no hipMalloc / hipFree.  When its master differs from the last one used on
+ * that GPU it first copies the tables in, which synchronises ctx's stream.
+ * Every call on one GPU shares the array: calls that overlap on one GPU
+ * (other streams, threads or contexts) race and must be serialised. */
+int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                           const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
+                           const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
+                           int32_t* status);
+
 /* Element matrices only: the outputs of volint + faceint that feed the Schur
  * step (kernel/volint.m:1-263, kernel/faceint.m:1-360), in the layouts that
  * kernel/schur_primal.m:5-11 receives them:
