@@ -49,9 +49,42 @@ uint64_t fnv(uint64_t h, const void* p, size_t n) {
 // table in it, and a content hash (FNV-1a over the sizes and every table).
 struct MasterImage {
   std::vector<char> host;
-  size_t off[11];
+  size_t off[13];
   uint64_t hash;
 };
+
+// AE k-step table of the elimination for nc components per node: an AE row
+// tile (16 rows of the (ncu, npf, nfe) trace rows) meets the faces of its
+// rows, and only the GK columns of those faces' nodes are nonzero
+// (kernel/faceint.m:354-358).  Layout: the column count of each tile, then
+// each tile's 4*KSMAX columns in k-step order (node fastest, then component),
+// 255 past the count.  KSMAX = ceil(max(npv, 2 npf) nc / 4), as in the kernel.
+std::vector<unsigned char> ae_table(int npv, int npf, int nfe, int ncu, int nc, const int32_t* perm) {
+  const int nh = ncu * npf * nfe, trh = (nh + 15) / 16;
+  const int nnmax = std::max(npv, 2 * npf), ksmax = (nnmax * nc + 3) / 4;
+  std::vector<unsigned char> t((size_t)trh * (1 + 4 * ksmax), 255);
+  // column indices are bytes: orders past the built configurations (npv nc
+  // above 254) get empty tables, which no kernel reads
+  if (npv * nc > 254) {
+    for (int rta = 0; rta < trh; ++rta) t[rta] = 0;
+    return t;
+  }
+  for (int rta = 0; rta < trh; ++rta) {
+    const int j0 = (16 * rta) / (ncu * npf);
+    const int j1 = std::min(nfe - 1, (16 * rta + 15) / (ncu * npf));
+    std::vector<int> nodes;
+    for (int j = j0; j <= j1; ++j)
+      for (int a = 0; a < npf; ++a) {
+        const int v = perm[a + npf * j];
+        if (std::find(nodes.begin(), nodes.end(), v) == nodes.end()) nodes.push_back(v);
+      }
+    const int nn = (int)nodes.size(), klen = nn * nc;
+    t[rta] = (unsigned char)std::min(klen, 255);
+    for (int idx = 0; idx < 4 * ksmax && idx < klen; ++idx)
+      t[trh + (size_t)rta * 4 * ksmax + idx] = (unsigned char)(nodes[idx % nn] + npv * (idx / nn));
+  }
+  return t;
+}
 
 int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   if (!m || !m->shapvt || !m->shapvg || !m->shapvgdotshapvl || !m->shapft || !m->shapfg || !m->shapfgdotshapfc ||
@@ -81,14 +114,18 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
     }
   for (size_t i = 0; i < n_perm; ++i)
     if (m->permgeom[i] < 0 || m->permgeom[i] >= npv) return fail(ctx, HDGX_ERR_ARG, "master: permgeom out of range");
+  const int ncu = nd + 2;
+  const std::vector<unsigned char> aek_u = ae_table(npv, npf, nfe, ncu, ncu, m->perm);
+  const std::vector<unsigned char> aek_uq = ae_table(npv, npf, nfe, ncu, ncu * (nd + 1), m->perm);
   const size_t nd64 = n_vt + n_vg + n_vv + n_ft + n_fg + n_ff + n_mv + n_mf;
-  img->host.assign(nd64 * 8 + (2 * n_perm + n_loc) * 4, 0);
+  img->host.assign(nd64 * 8 + (2 * n_perm + n_loc) * 4 + aek_u.size() + aek_uq.size(), 0);
   size_t at = 0;
   int k = 0;
   auto put = [&](const void* src, size_t n) { std::memcpy(img->host.data() + at, src, n); img->off[k++] = at; at += n; };
   put(m->shapvt, n_vt * 8); put(m->shapvg, n_vg * 8); put(m->shapvgdotshapvl, n_vv * 8); put(m->shapft, n_ft * 8);
   put(m->shapfg, n_fg * 8); put(m->shapfgdotshapfc, n_ff * 8); put(m->shapmv, n_mv * 8); put(m->shapmf, n_mf * 8);
   put(m->perm, n_perm * 4); put(m->permgeom, n_perm * 4); put(loc.data(), n_loc * 4);
+  put(aek_u.data(), aek_u.size()); put(aek_uq.data(), aek_uq.size());
   return HDGX_OK;
 }
 
@@ -99,6 +136,7 @@ void bind_master(const hdgx_master* m, const MasterImage& img, char* base, Maste
   mb->tab.shapft = (const double*)dp(3); mb->tab.shapfg = (const double*)dp(4); mb->tab.sff = (const double*)dp(5);
   mb->tab.shapmv = (const double*)dp(6); mb->tab.shapmf = (const double*)dp(7);
   mb->tab.perm = (const int32_t*)dp(8); mb->tab.permgeom = (const int32_t*)dp(9); mb->tab.loc = (const int32_t*)dp(10);
+  mb->tab.aek[0] = (const unsigned char*)dp(11); mb->tab.aek[1] = (const unsigned char*)dp(12);
   mb->nd = m->nd; mb->porder = m->porder; mb->npv = m->npv; mb->npf = m->npf; mb->nfe = m->nfe;
   mb->ngv = m->ngv; mb->ngf = m->ngf;
 }
@@ -271,7 +309,7 @@ int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, con
                 nullptr};
     SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
                  nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
-                 status ? status + e0 : nullptr, ctx->prec, nco};
+                 status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0]};
     const int rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
       using Cc = decltype(cfg);
       int r = launch_quad<Cc>(ctx, mb, ap, qa);

@@ -6,6 +6,7 @@
 #include "quad_kernel.hip"
 #include "getq_kernel.hip"
 #include "schur_mfma.hip"
+#include "schur64.hip"
 #include "schur_lds.hip"
 #include "seg_kernels.hip"
 
@@ -31,6 +32,12 @@ hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
 
 template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
+  if constexpr (S64Shape<C>::USE) {
+    if (a.prec == 0) {
+      hipLaunchKernelGGL(schur64_kernel<C>, dim3((unsigned)a.ne), dim3(64), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if constexpr (SchurLdsShape<C>::USE) {
     if (a.prec == 0) {
       hipLaunchKernelGGL(schur_lds_kernel<C>, dim3((unsigned)a.ne), dim3(SCHUR_LDS_THREADS), 0, s, a);
@@ -53,7 +60,7 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
   if constexpr (SchurLdsShape<C>::USE)
     hipLaunchKernelGGL(schur_seg_lds_kernel<C>, dim3((unsigned)t.total), dim3(SCHUR_LDS_THREADS), 0, s, t);
   else
-    hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm);
+    hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm, m.aek[C::UQ ? 1 : 0]);
   return hipGetLastError();
 }
 

@@ -42,6 +42,11 @@ struct DevMaster {
   const int32_t* perm;            // (npf, nfe)
   const int32_t* permgeom;        // (npf, nfe)
   const int32_t* loc;             // (npv, nfe): position of volume node v on face j, or -1
+  // AE k-step tables of the face-sparse GK (u only: [0], u and q: [1]): per
+  // AE row tile of 16 rows its count of nonzero GK columns, then per tile
+  // 4*KSMAX column indices (255 past the count), KSMAX = ceil(max(npv,
+  // 2 npf) * nc / 4)
+  const unsigned char* aek[2];
 };
 
 // Physics parameters passed by value.
@@ -94,6 +99,7 @@ struct SchurArgs {
   int32_t* status;
   int32_t prec;                    // 0 fp64, 1 fp32 mfma + fp64 refinement
   int32_t nco;                     // DUDG / DUDG_DUH rows kept per node: nc, ncu or 0
+  const unsigned char* aek;        // AE k-step table (DevMaster::aek) when GK is face sparse, else null
 };
 
 // One request's element range inside a batched launch: its arrays and the

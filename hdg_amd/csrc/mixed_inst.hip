@@ -5,6 +5,7 @@
 #include "hdgx_launch.h"
 #include "quad_kernel.hip"
 #include "schur_mfma.hip"
+#include "schur64.hip"
 #include "schur_lds.hip"
 #include "seg_kernels.hip"
 #include "mixed_kernels.hip"

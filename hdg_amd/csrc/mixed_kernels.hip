@@ -84,6 +84,7 @@ HDGX_DEV void schur_mixed_elem(const MixedTable& T, const ElemSeg& sg, int64_t e
   A.status = sg.status ? sg.status + e : nullptr;
   A.prec = 0;
   A.nco = C::NC;
+  A.aek = nullptr;
   schur_mfma_body<C>(A, T.masters[P].perm, 0, lds);
 }
 

@@ -99,16 +99,27 @@ HDGX_DEV SchurArgs seg_schur_args(const SegTable& S, int64_t w) {
   A.status = sg.status ? sg.status + e : nullptr;
   A.prec = 0;
   A.nco = C::NC;
+  A.aek = nullptr;
   return A;
 }
 
+// the elimination body of hdgx_elemental's FP64 path for this configuration
 template <class C>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MfmaShape<C>::WPE, MfmaShape<C>::WPE))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm) {
+constexpr int seg_wpe() { return S64Shape<C>::USE ? S64Shape<C>::WPE : MfmaShape<C>::WPE; }
+
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>(), seg_wpe<C>()))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm, const unsigned char* __restrict__ aek) {
   const int64_t w = blockIdx.x;
   if (w >= S.total) return;
-  const SchurArgs A = seg_schur_args<C>(S, w);
-  __shared__ SchurLds<C> lds;
-  schur_mfma_body<C>(A, perm, 0, lds);
+  SchurArgs A = seg_schur_args<C>(S, w);
+  if constexpr (S64Shape<C>::USE) {
+    A.aek = aek;
+    __shared__ S64Lds<C> lds;
+    schur64_body<C>(A, 0, lds);
+  } else {
+    __shared__ SchurLds<C> lds;
+    schur_mfma_body<C>(A, perm, 0, lds);
+  }
 }
 
 // the configurations hdgx_elemental eliminates in LDS (SchurLdsShape<C>::USE)
