@@ -19,6 +19,7 @@
 #include "hdgx_kernels.h"
 
 #include "hdgx_launch.h"
+#include "hdgx_shapes.h"
 
 using namespace hdgx;
 using hdgx_detail::fail;
@@ -201,11 +202,11 @@ int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) 
 
 // ---------------------------------------------------------------- dispatch
 template <class C>
-int launch_quad(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const QuadArgs& qa) {
+int launch_quad(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const QuadArgs& qa, bool tile = false) {
   if (mb.npv != C::NPV || mb.npf != C::NPF || mb.ngv != C::NGV || mb.ngf != C::NGF)
     return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled pgauss = 2p configuration");
   if (qa.ne == 0) return HDGX_OK;
-  HIP_TRY(ctx, quad_launch<C>(ctx->stream, qa, mb.tab, ap));
+  HIP_TRY(ctx, quad_launch<C>(ctx->stream, qa, mb.tab, ap, tile));
   return HDGX_OK;
 }
 
@@ -228,9 +229,9 @@ int launch_getq(hdgx_ctx* ctx, const MasterBuf& mb, const GetqArgs& ga) {
 }
 
 template <class C>
-int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa, const int32_t* perm) {
+int launch_schur(hdgx_ctx* ctx, const SchurArgs& sa, const int32_t* perm, bool tile = false) {
   if (sa.ne == 0) return HDGX_OK;
-  HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa, perm));
+  HIP_TRY(ctx, schur_launch<C>(ctx->stream, sa, perm, tile));
   return HDGX_OK;
 }
 
@@ -251,10 +252,16 @@ int with_family(int phys, int nd, F&& f) {
   return -1;
 }
 
-size_t per_element_words(int nd, int npv, int npf, int nfe, int ncu, int nc, bool uq) {
-  const int ndf = npf * nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
-  return (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
-         (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
+// per-element words of the element-matrix workspace of configuration
+// (phys, nd, p), the larger of its reference and tile layouts (hdgx_shapes.h);
+// 0 when no kernel is built for it
+size_t ws_words_of(int phys, int nd, int p) {
+  size_t w = 0;
+  with_cfg(phys, nd, p, [&](auto cfg) {
+    w = ws_words<decltype(cfg)>();
+    return 0;
+  });
+  return w;
 }
 
 int porder_of(int nd, int npv) {
@@ -276,45 +283,35 @@ int ensure_ws(hdgx_ctx* ctx, size_t bytes) {
 }
 
 // hdgx_elemental's element loop over chunks of `chunk` elements, the element
-// matrices of each chunk in the workspace at ws (per_element_words * chunk
+// matrices of each chunk in the workspace at ws (ws_words_of * chunk
 // doubles): the quadrature kernel writes them, the elimination reads them.
+// The FP64 eliminations of up to three row tiles take F, GK and H in their
+// fragment order (the tile layout of hdgx_shapes.h).
 int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const hdgx_master* master,
                        const hdgx_app* app, int64_t ne, const double* dgnodes, const int32_t* bf, const double* udg,
                        const double* uh, const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,
                        int32_t* status, double* ws, int64_t chunk) {
   const int nco = app->getdqdg == 1 ? app->nc : (app->getdqdg == 0 ? app->ncu : 0);
-  const int nd = master->nd, npv = master->npv, ncu = app->ncu, nc = app->nc;
+  const int nd = master->nd, npv = master->npv, nc = app->nc;
   const size_t ldo = (size_t)npv * nco;
-  const int ndf = master->npf * master->nfe, nu = npv * ncu, nh = ncu * ndf, ncol = npv * nc;
+  const int ndf = master->npf * master->nfe, nh = app->ncu * ndf;
   const int p = porder_of(nd, npv);
   const bool uq = app->localsolve != 0;
   for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
     const int64_t n = std::min(chunk, ne - e0);
-    double* w = ws;
-    double* BD = w; w += (size_t)nu * ncol * n;
-    double* F = w; w += (size_t)nu * nh * n;
-    double* GK = w; w += (size_t)nh * ncol * n;
-    double* H = w; w += (size_t)nh * nh * n;
-    double* Ru = w; w += (size_t)nu * n;
-    double* Rh = w; w += (size_t)nh * n;
-    double *M = nullptr, *C = nullptr, *E = nullptr, *Rq = nullptr;
-    if (uq) {
-      M = w; w += (size_t)npv * npv * n;
-      C = w; w += (size_t)npv * npv * nd * n;
-      E = w; w += (size_t)npv * ndf * nd * n;
-      Rq = w; w += (size_t)npv * ncu * nd * n;
-    }
-    QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe, udg + (size_t)e0 * npv * nc,
-                uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr, BD, F, GK, H, Ru, Rh, M, C, E, Rq,
-                nullptr};
-    SchurArgs sa{n, M, C, E, BD, F, GK, H, Rq, Ru, Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
-                 nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
-                 status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0]};
     const int rc = with_cfg(app->physics, nd, p, [&](auto cfg) {
       using Cc = decltype(cfg);
-      int r = launch_quad<Cc>(ctx, mb, ap, qa);
+      const bool tile = ctx->prec == HDGX_PREC_FP64 && WsLayout<Cc>::TILE_OK;
+      const WsPtrs W = ws_carve<Cc>(ws, n, tile);
+      QuadArgs qa{n, dgnodes + (size_t)e0 * npv * app->ncd, bf + (size_t)e0 * master->nfe,
+                  udg + (size_t)e0 * npv * nc, uh + (size_t)e0 * nh, sh ? sh + (size_t)e0 * npv * nc : nullptr,
+                  W.BD, W.F, W.GK, W.H, W.Ru, W.Rh, W.M, W.C, W.E, W.Rq, nullptr};
+      SchurArgs sa{n, W.M, W.C, W.E, W.BD, W.F, W.GK, W.H, W.Rq, W.Ru, W.Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
+                   nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
+                   status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0]};
+      int r = launch_quad<Cc>(ctx, mb, ap, qa, tile);
       if (r) return r;
-      return launch_schur<Cc>(ctx, sa, mb.tab.perm);
+      return launch_schur<Cc>(ctx, sa, mb.tab.perm, tile);
     });
     if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
     if (rc) return rc;
@@ -746,8 +743,8 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
   if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
   if (ne == 0) return HDGX_OK;
   // per-element workspace for the Schur inputs
-  const size_t per = per_element_words(master->nd, master->npv, master->npf, master->nfe, app->ncu, app->nc,
-                                       app->localsolve != 0);
+  const size_t per = ws_words_of(app->physics, master->nd, porder_of(master->nd, master->npv));
+  if (per == 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
   const int64_t chunk = ws_chunk(ne, per * 8, ctx->ws_cap);
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   return elemental_launches(ctx, *mb, ap, master, app, ne, dgnodes, bf, udg, uh, sh, ae, fe, dudg, dudg_duh, status,
@@ -780,8 +777,8 @@ int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_
   DevApp ap;
   if ((rc = make_app(ctx, master, app, &ap))) return rc;
   if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
-  const size_t per = 8 * per_element_words(master->nd, master->npv, master->npf, master->nfe, app->ncu, app->nc,
-                                           app->localsolve != 0);
+  const size_t per = 8 * ws_words_of(app->physics, master->nd, porder_of(master->nd, master->npv));
+  if (per == 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
   const size_t ws_bytes = NOALLOC_BYTES - NOALLOC_TABLE_BYTES;
   if (per > ws_bytes) return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: one element's matrices exceed the scratch");
   if (ne == 0) return HDGX_OK;
@@ -844,35 +841,26 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
   DevApp* dapps = reinterpret_cast<DevApp*>(dsegs + nseg);
   HIP_TRY(ctx, hipMemcpyAsync(dsegs, tab.data(), sizeof(ElemSeg) * nseg, hipMemcpyHostToDevice, ctx->stream));
   HIP_TRY(ctx, hipMemcpyAsync(dapps, apps.data(), sizeof(DevApp) * nseg, hipMemcpyHostToDevice, ctx->stream));
-  const size_t per = (size_t)nu * ncol + (size_t)nu * nh + (size_t)nh * ncol + (size_t)nh * nh + nu + nh +
-                     (uq ? (size_t)npv * npv + (size_t)npv * npv * nd + (size_t)npv * ndf * nd + (size_t)npv * ncu * nd : 0);
+  const size_t per = ws_words_of(a0->physics, nd, p);
+  if (per == 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
   const int64_t chunk = ws_chunk(total, per * 8, ctx->ws_cap);
   if ((rc = ensure_ws(ctx, per * 8 * (size_t)chunk))) return rc;
   for (int64_t g0 = 0; g0 < total; g0 += chunk) {
     const int64_t n = std::min(chunk, total - g0);
-    double* w = static_cast<double*>(ctx->ws);
-    SegTable t{};
-    t.segs = dsegs;
-    t.apps = dapps;
-    t.nseg = nseg;
-    t.g0 = g0;
-    t.total = n;
-    t.BD = w; w += (size_t)nu * ncol * n;
-    t.F = w; w += (size_t)nu * nh * n;
-    t.GK = w; w += (size_t)nh * ncol * n;
-    t.H = w; w += (size_t)nh * nh * n;
-    t.Ru = w; w += (size_t)nu * n;
-    t.Rh = w; w += (size_t)nh * n;
-    if (uq) {
-      t.M = w; w += (size_t)npv * npv * n;
-      t.C = w; w += (size_t)npv * npv * nd * n;
-      t.E = w; w += (size_t)npv * ndf * nd * n;
-      t.Rq = w; w += (size_t)npv * ncu * nd * n;
-    }
     rc = with_cfg(a0->physics, nd, p, [&](auto cfg) {
       using Cc = decltype(cfg);
       if (mb->npv != Cc::NPV || mb->npf != Cc::NPF || mb->ngv != Cc::NGV || mb->ngf != Cc::NGF)
         return fail(ctx, HDGX_ERR_UNSUPPORTED, "master quadrature does not match the compiled configuration");
+      // batched launches take the tile layout where it exists (seg_kernels.hip)
+      const WsPtrs W = ws_carve<Cc>(static_cast<double*>(ctx->ws), n, WsLayout<Cc>::TILE_OK);
+      SegTable t{};
+      t.segs = dsegs;
+      t.apps = dapps;
+      t.nseg = nseg;
+      t.g0 = g0;
+      t.total = n;
+      t.BD = W.BD; t.F = W.F; t.GK = W.GK; t.H = W.H; t.Ru = W.Ru; t.Rh = W.Rh;
+      t.M = W.M; t.C = W.C; t.E = W.E; t.Rq = W.Rq;
       HIP_TRY(ctx, seg_launch<Cc>(ctx->stream, t, mb->tab, mb->tab.perm));
       return (int)HDGX_OK;
     });
@@ -1043,9 +1031,11 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
     if (P->n[p] == 0) continue;
     int s0 = 0;
     for (int s = 0; s < nseg; ++s) if (pof[s] == p) { s0 = s; break; }
-    const hdgx_master* m = masters[s0];
+    (void)s0;
     off[p] = words;
-    words += per_element_words(P->nd, m->npv, m->npf, m->nfe, a->ncu, a->nc, uq) * (size_t)P->n[p];
+    const size_t w = ws_words_of(P->phys, P->nd, p);
+    if (w == 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no kernel for this order");
+    words += w * (size_t)P->n[p];
   }
   HIP_TRY(ctx, hipMalloc(&P->ws, std::max<size_t>(words, 1) * 8));
   MixedTable mt{};
@@ -1061,8 +1051,7 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
     if (P->n[p] == 0) continue;
     int s0 = 0;
     for (int s = 0; s < nseg; ++s) if (pof[s] == p) { s0 = s; break; }
-    const hdgx_master* m = masters[s0];
-    const int npv = m->npv, ndf = m->npf * m->nfe, nu = npv * a->ncu, nh = a->ncu * ndf, ncol = npv * a->nc;
+    (void)s0;
     const int64_t n = P->n[p];
     double* w = static_cast<double*>(P->ws) + off[p];
     SegTable& t = P->order_tab[p];
@@ -1071,18 +1060,16 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
     t.nseg = count[p];
     t.g0 = 0;
     t.total = n;
-    t.BD = w; w += (size_t)nu * ncol * n;
-    t.F = w; w += (size_t)nu * nh * n;
-    t.GK = w; w += (size_t)nh * ncol * n;
-    t.H = w; w += (size_t)nh * nh * n;
-    t.Ru = w; w += (size_t)nu * n;
-    t.Rh = w; w += (size_t)nh * n;
-    if (uq) {
-      t.M = w; w += (size_t)npv * npv * n;
-      t.C = w; w += (size_t)npv * npv * P->nd * n;
-      t.E = w; w += (size_t)npv * ndf * P->nd * n;
-      t.Rq = w; w += (size_t)npv * a->ncu * P->nd * n;
-    }
+    // per-order batched launches take the tile layout where it exists, the
+    // fused persistent kernels the reference layout
+    const int rcw = with_cfg(P->phys, P->nd, p, [&](auto cfg) {
+      using Cc = decltype(cfg);
+      const WsPtrs W = ws_carve<Cc>(w, n, mode != HDGX_PLAN_FUSED && WsLayout<Cc>::TILE_OK);
+      t.BD = W.BD; t.F = W.F; t.GK = W.GK; t.H = W.H; t.Ru = W.Ru; t.Rh = W.Rh;
+      t.M = W.M; t.C = W.C; t.E = W.E; t.Rq = W.Rq;
+      return 0;
+    });
+    if (rcw < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no kernel for this order");
     mt.masters[p] = P->tab[p];
     mt.BD[p] = t.BD; mt.F[p] = t.F; mt.GK[p] = t.GK; mt.H[p] = t.H; mt.Ru[p] = t.Ru; mt.Rh[p] = t.Rh;
     mt.M[p] = t.M; mt.C[p] = t.C; mt.E[p] = t.E; mt.Rq[p] = t.Rq;

@@ -13,9 +13,21 @@
 namespace hdgx {
 
 template <class C>
-hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
-  hipLaunchKernelGGL(quad_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap, bool tile) {
+  if constexpr (WsLayout<C>::TILE_OK) {
+    if (tile) {
+      hipLaunchKernelGGL((quad_kernel<C, true>), dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+      return hipGetLastError();
+    }
+  }
+  if (tile) return hipErrorInvalidValue;
+  hipLaunchKernelGGL((quad_kernel<C, false>), dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
   return hipGetLastError();
+}
+
+template <class C>
+size_t ws_words() {
+  return WsLayout<C>::max_words();
 }
 
 template <class C>
@@ -31,13 +43,16 @@ hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
 }
 
 template <class C>
-hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm) {
+hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, bool tile) {
   if constexpr (S64Shape<C>::USE) {
     if (a.prec == 0) {
-      hipLaunchKernelGGL(schur64_kernel<C>, dim3((unsigned)a.ne), dim3(64), 0, s, a);
+      if (tile && !a.aek) return hipErrorInvalidValue;
+      if (tile) hipLaunchKernelGGL((schur64_kernel<C, true>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+      else hipLaunchKernelGGL((schur64_kernel<C, false>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
       return hipGetLastError();
     }
   }
+  if (tile) return hipErrorInvalidValue;
   if constexpr (SchurLdsShape<C>::USE) {
     if (a.prec == 0) {
       hipLaunchKernelGGL(schur_lds_kernel<C>, dim3((unsigned)a.ne), dim3(SCHUR_LDS_THREADS), 0, s, a);
@@ -65,12 +80,13 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
 }
 
 using ThisCfg = Cfg<HDGX_PHYS, HDGX_ND, HDGX_P>;
-template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
+template hipError_t quad_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&, bool);
+template size_t ws_words<ThisCfg>();
 template hipError_t resid_launch<ThisCfg>(hipStream_t, const QuadArgs&, const DevMaster&, const DevApp&);
 #if HDGX_PHYS == 0
 template hipError_t getq_launch<ThisCfg>(hipStream_t, const GetqArgs&, const DevMaster&);
 #endif
-template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*);
+template hipError_t schur_launch<ThisCfg>(hipStream_t, const SchurArgs&, const int32_t*, bool);
 template hipError_t seg_launch<ThisCfg>(hipStream_t, const SegTable&, const DevMaster&, const int32_t*);
 
 }  // namespace hdgx

@@ -18,7 +18,10 @@
 
 namespace hdgx {
 
-template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
+// tile: write F, GK and H in the elimination's fragment order (WsLayout in
+// hdgx_shapes.h) for a schur_launch with tile set; otherwise the reference layout
+template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap,
+                                          bool tile);
 // residual only (Ru, Rh with hdg_residual's sign, Rq)
 template <class C> hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap);
 // the MEX files' in-place side effects (one launch after the elimination)
@@ -28,17 +31,21 @@ hipError_t inplace_launch(hipStream_t s, const InplaceArgs& a);
 template <class C> hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t);
 // prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 MFMA Gauss-Jordan (with D'^-1) + FP64
 // refinement.
-// perm (device, may be null) marks GK as face sparse.
-template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm);
+// perm (device, may be null) marks GK as face sparse; tile: F, GK and H in the
+// fragment order (FP64, configurations with WsLayout<C>::TILE_OK)
+template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, bool tile);
+// per-element words of the element-matrix workspace (the larger of its two layouts)
+template <class C> size_t ws_words();
 // batched launches over a segment table (FP64 MFMA elimination)
 template <class C> hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, const int32_t* perm);
 
 #define HDGX_DECLARE(PH, ND, P)                                                                          \
   extern template hipError_t quad_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
-                                                         const DevApp&);                                  \
+                                                         const DevApp&, bool);                            \
+  extern template size_t ws_words<Cfg<PH, ND, P>>();                                                      \
   extern template hipError_t resid_launch<Cfg<PH, ND, P>>(hipStream_t, const QuadArgs&, const DevMaster&, \
                                                           const DevApp&);                                 \
-  extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&, const int32_t*); \
+  extern template hipError_t schur_launch<Cfg<PH, ND, P>>(hipStream_t, const SchurArgs&, const int32_t*, bool); \
   extern template hipError_t seg_launch<Cfg<PH, ND, P>>(hipStream_t, const SegTable&, const DevMaster&,  \
                                                         const int32_t*);
 HDGX_FOR_EACH_CFG(HDGX_DECLARE)

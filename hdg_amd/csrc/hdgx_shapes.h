@@ -1,0 +1,106 @@
+// Compile-time shapes shared by the kernels and the host dispatch: the FP64
+// elimination for one to three row tiles (schur64.hip) and the layout of the
+// element-matrix workspace between the quadrature and elimination kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "hdgx_kernels.h"
+
+namespace hdgx {
+
+template <class C>
+struct S64Shape {
+  static constexpr int NU = C::NU, NH = C::NH, NPV = C::NPV, NCU = C::NCU, ND = C::ND, NC = C::NC;
+  static constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NCOL = C::NCOL;
+  static constexpr int RT = (NU + 15) / 16, R = 16 * RT;
+  static constexpr int DU = NU;                  // padding column with du's right-hand side
+  static constexpr int CTD = DU / 16;            // its tile
+  static constexpr int TRH = (NH + 15) / 16;     // RHS tiles = AE row tiles
+  static constexpr int CT = RT + TRH;            // column tiles of the system
+  static constexpr bool USE = RT <= 3 && NU % 4 == 0 && NH % 4 == 0 && NU < R;
+  static constexpr int KB = NPV * NCU * ND;      // columns of B (u,q)
+  static constexpr int KBS = (KB + 3) / 4;
+  static constexpr int KSV = (NPV + 3) / 4;      // k-steps of the q-row products
+  // AE k-step table (same constants as the host's ae_table)
+  static constexpr int NNMAX = NPV > 2 * NPF ? NPV : 2 * NPF;
+  static constexpr int KSMAX = (NNMAX * NC + 3) / 4;
+  // k-steps of a face-sparse AE row tile at most: its rows meet faces
+  // j0..j1, so at most min(NPV, faces * NPF) nodes
+  static constexpr int ksp_tile(int rta) {
+    const int j0 = (16 * rta) / (NCU * NPF);
+    int j1 = (16 * rta + 15) / (NCU * NPF);
+    if (j1 > NFE - 1) j1 = NFE - 1;
+    int nn = (j1 - j0 + 1) * NPF;
+    if (nn > NPV) nn = NPV;
+    return (nn * NC + 3) / 4;
+  }
+  static constexpr int ksp() {
+    int m = 0;
+    for (int r = 0; r < TRH; ++r) m = ksp_tile(r) > m ? ksp_tile(r) : m;
+    return m;
+  }
+  static constexpr int KSP = ksp();
+  // GK's A fragments stay in registers across the RHS tiles when they fit
+  static constexpr bool GKREG = KSP * TRH <= 16;
+  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
+};
+
+// Per-element words of the element-matrix workspace in hdgx_elemental and the
+// batched launches.  The reference layout keeps every array as
+// kernel/schur_primal.m receives it; the tile layout (FP64 configurations of
+// schur64) stores F, GK and H in the elimination's MFMA fragment order, one
+// 64-lane vector per (tile, register) so that every load of the elimination
+// reads 512 contiguous bytes:
+//   F  (NU/4, TRH) vectors: lane 16g + c of vector (ri, tb) = F(u row 4 ri + g,
+//      trace column j = 16 tb + c);
+//   GK (KSP, TRH) vectors: lane 16g + c of vector (s, rta) = GK(row 16 rta + c,
+//      column of k-step s, lane group g of the AE table), only the face-sparse
+//      columns (faceint.m:354-358);
+//   H  (TRH, NH/4) vectors: lane 16g + c of vector (tb, rh) = H(row 4 rh + g,
+//      trace column j = 16 tb + c);
+// trace column j = jf + NDF k is H's / F's column k + NCU jf, zero past NH.
+// The other arrays (BD, Ru, Rh, M, C, E, Rq) keep the reference layout.
+template <class C>
+struct WsLayout {
+  using S = S64Shape<C>;
+  static constexpr bool TILE_OK = S::USE;
+  static constexpr size_t NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, ND = C::ND;
+  static constexpr size_t bd = NU * NCOL, ru = NU, rh = NH;
+  static constexpr size_t m = C::UQ ? NPV * NPV : 0, cm = C::UQ ? NPV * NPV * ND : 0;
+  static constexpr size_t em = C::UQ ? NPV * C::NDF * ND : 0, rq = C::UQ ? NPV * C::NCU * ND : 0;
+  static constexpr size_t f(bool tile) { return tile ? (NU / 4) * S::TRH * 64 : NU * NH; }
+  static constexpr size_t gk(bool tile) { return tile ? (size_t)S::KSP * S::TRH * 64 : NH * NCOL; }
+  static constexpr size_t h(bool tile) { return tile ? S::TRH * (NH / 4) * 64 : NH * NH; }
+  static constexpr size_t words(bool tile) { return bd + f(tile) + gk(tile) + h(tile) + ru + rh + m + cm + em + rq; }
+  static constexpr size_t max_words() {
+    return TILE_OK && words(true) > words(false) ? words(true) : words(false);
+  }
+};
+
+// the workspace of n elements carved into its arrays (each element-major)
+struct WsPtrs {
+  double *BD, *F, *GK, *H, *Ru, *Rh, *M, *C, *E, *Rq;
+};
+template <class C>
+__host__ __device__ inline WsPtrs ws_carve(double* w, int64_t n, bool tile) {
+  using L = WsLayout<C>;
+  WsPtrs p{};
+  p.BD = w; w += L::bd * n;
+  p.F = w; w += L::f(tile) * n;
+  p.GK = w; w += L::gk(tile) * n;
+  p.H = w; w += L::h(tile) * n;
+  p.Ru = w; w += L::ru * n;
+  p.Rh = w; w += L::rh * n;
+  if (C::UQ) {
+    p.M = w; w += L::m * n;
+    p.C = w; w += L::cm * n;
+    p.E = w; w += L::em * n;
+    p.Rq = w;
+  }
+  return p;
+}
+
+}  // namespace hdgx

@@ -26,6 +26,7 @@
 
 #include "hdgx_device.h"
 #include "hdgx_kernels.h"
+#include "hdgx_shapes.h"
 
 namespace hdgx {
 
@@ -265,7 +266,7 @@ struct QuadLds {
 // RO: the residual only (hdg_residual.m:1-57: Ru, Rq and Rh with that
 // function's sign, +shapfg (fbou | fhat), hdg_residual.m:286-290), no
 // Jacobians and no element matrices.
-template <class C, bool RO = false>
+template <class C, bool RO = false, bool TILE = false>
 HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e,
                         QuadLds<C, RO>& L) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
@@ -617,52 +618,100 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     Rh[idx] = RO ? v : -v;
   }
   if constexpr (!RO) {
-    // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327)
-    double* F = A.F + e * (int64_t)NU * NH;
-    for (int idx = tid; idx < NU * NH; idx += NT) {
-      const int va = idx % NPV;
-      int rest = idx / NPV;
-      const int c = rest % NCU;
-      rest /= NCU;
-      const int k = rest % NCU;
-      const int bj = rest / NCU, j = bj / NPF, b = bj - j * NPF;
+    // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327); F(va, c, k, bj) is u row
+    // va + NPV c, trace column k + NCU bj
+    auto f_entry = [&](int va, int c, int k, int bj) {
+      const int j = bj / NPF, b = bj - j * NPF;
       const int la = sLoc[va + NPV * j];
       double v = 0.0;
       if (la >= 0)
   #pragma unroll
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
-      F[idx] = v;
+      return v;
+    };
+    if constexpr (TILE) {
+      // the elimination's fragment order (hdgx_shapes.h, WsLayout)
+      using S = S64Shape<C>;
+      constexpr int NV = (NU / 4) * S::TRH * 64;
+      double* F = A.F + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int ln = idx & 63, rest = idx >> 6, tb = rest % S::TRH, ri = rest / S::TRH;
+        const int row = 4 * ri + (ln >> 4), jj = 16 * tb + (ln & 15);
+        F[idx] = jj < NH ? f_entry(row % NPV, row / NPV, jj / NDF, jj % NDF) : 0.0;
+      }
+    } else {
+      double* F = A.F + e * (int64_t)NU * NH;
+      for (int idx = tid; idx < NU * NH; idx += NT) {
+        const int va = idx % NPV;
+        int rest = idx / NPV;
+        const int c = rest % NCU;
+        rest /= NCU;
+        const int k = rest % NCU;
+        F[idx] = f_entry(va, c, k, rest / NCU);
+      }
     }
-    // ---- 3e. GK (ncu,ndf,npv,nc) (faceint.m:342-358)
-    double* GK = A.GK + e * (int64_t)NH * C::NCOL;
-    for (int idx = tid; idx < NH * C::NCOL; idx += NT) {
-      const int c = idx % NCU;
-      int rest = idx / NCU;
-      const int bj = rest % NDF;
-      rest /= NDF;
-      const int va = rest % NPV, k = rest / NPV;
+    // ---- 3e. GK (ncu,ndf,npv,nc) (faceint.m:342-358); GK(c, bj, va, k) is row
+    // c + NCU bj, column va + NPV k
+    auto gk_entry = [&](int c, int bj, int va, int k) {
       const int j = bj / NPF, b = bj - j * NPF;
       const int la = sLoc[va + NPV * j];
       double v = 0.0;
       if (la >= 0)
   #pragma unroll
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
-      GK[idx] = v;
+      return v;
+    };
+    if constexpr (TILE) {
+      // A fragments of the AE product over the face-sparse columns (the
+      // master's AE k-step table)
+      using S = S64Shape<C>;
+      constexpr int NV = S::KSP * S::TRH * 64;
+      const unsigned char* tab = T.aek[UQ ? 1 : 0];
+      double* GK = A.GK + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int ln = idx & 63, rest = idx >> 6, rta = rest % S::TRH, s = rest / S::TRH;
+        const int row = 16 * rta + (ln & 15);
+        const int kcol = tab[S::TRH + rta * 4 * S::KSMAX + 4 * s + (ln >> 4)];
+        GK[idx] = (row < NH && kcol != 255) ? gk_entry(row % NCU, row / NCU, kcol % NPV, kcol / NPV) : 0.0;
+      }
+    } else {
+      double* GK = A.GK + e * (int64_t)NH * C::NCOL;
+      for (int idx = tid; idx < NH * C::NCOL; idx += NT) {
+        const int c = idx % NCU;
+        int rest = idx / NCU;
+        const int bj = rest % NDF;
+        rest /= NDF;
+        GK[idx] = gk_entry(c, bj, rest % NPV, rest / NPV);
+      }
     }
-    // ---- 3f. H (ncu,ndf,ncu,ndf), block diagonal per face (faceint.m:347-359)
-    double* H = A.H + e * (int64_t)NH * NH;
-    for (int idx = tid; idx < NH * NH; idx += NT) {
-      const int c = idx % NCU;
-      int rest = idx / NCU;
-      const int aj = rest % NDF;
-      rest /= NDF;
-      const int k = rest % NCU, bj = rest / NCU;
+    // ---- 3f. H (ncu,ndf,ncu,ndf), block diagonal per face (faceint.m:347-359);
+    // H(c, aj, k, bj) is row c + NCU aj, column k + NCU bj
+    auto h_entry = [&](int c, int aj, int k, int bj) {
       const int j = aj / NPF, a = aj - j * NPF, j2 = bj / NPF, b = bj - j2 * NPF;
       double v = 0.0;
       if (j == j2)
   #pragma unroll
         for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
-      H[idx] = v;
+      return v;
+    };
+    if constexpr (TILE) {
+      using S = S64Shape<C>;
+      constexpr int NHV = NH / 4, NV = S::TRH * NHV * 64;
+      double* H = A.H + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int ln = idx & 63, rest = idx >> 6, rih = rest % NHV, tb = rest / NHV;
+        const int row = 4 * rih + (ln >> 4), jj = 16 * tb + (ln & 15);
+        H[idx] = jj < NH ? h_entry(row % NCU, row / NCU, jj / NDF, jj % NDF) : 0.0;
+      }
+    } else {
+      double* H = A.H + e * (int64_t)NH * NH;
+      for (int idx = tid; idx < NH * NH; idx += NT) {
+        const int c = idx % NCU;
+        int rest = idx / NCU;
+        const int aj = rest % NDF;
+        rest /= NDF;
+        H[idx] = h_entry(c, aj, rest % NCU, rest / NCU);
+      }
     }
   }
   if constexpr (UQ) {
@@ -717,10 +766,10 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
 }
 
-template <class C>
+template <class C, bool TILE = false>
 __global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C> lds;
-  quad_body<C>(A, T, ap, blockIdx.x, lds);
+  quad_body<C, false, TILE>(A, T, ap, blockIdx.x, lds);
 }
 
 // residual only: Ru, Rh (hdg_residual.m's sign), Rq

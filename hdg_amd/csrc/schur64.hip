@@ -27,45 +27,9 @@
 
 #include "hdgx_device.h"
 #include "hdgx_kernels.h"
+#include "hdgx_shapes.h"
 
 namespace hdgx {
-
-template <class C>
-struct S64Shape {
-  static constexpr int NU = C::NU, NH = C::NH, NPV = C::NPV, NCU = C::NCU, ND = C::ND, NC = C::NC;
-  static constexpr int NDF = C::NDF, NPF = C::NPF, NFE = C::NFE, NCOL = C::NCOL;
-  static constexpr int RT = (NU + 15) / 16, R = 16 * RT;
-  static constexpr int DU = NU;                  // padding column with du's right-hand side
-  static constexpr int CTD = DU / 16;            // its tile
-  static constexpr int TRH = (NH + 15) / 16;     // RHS tiles = AE row tiles
-  static constexpr int CT = RT + TRH;            // column tiles of the system
-  static constexpr bool USE = RT <= 3 && NU % 4 == 0 && NU < R;
-  static constexpr int KB = NPV * NCU * ND;      // columns of B (u,q)
-  static constexpr int KBS = (KB + 3) / 4;
-  static constexpr int KSV = (NPV + 3) / 4;      // k-steps of the q-row products
-  // AE k-step table (same constants as the host's ae_table)
-  static constexpr int NNMAX = NPV > 2 * NPF ? NPV : 2 * NPF;
-  static constexpr int KSMAX = (NNMAX * NC + 3) / 4;
-  // k-steps of a face-sparse AE row tile at most: its rows meet faces
-  // j0..j1, so at most min(NPV, faces * NPF) nodes
-  static constexpr int ksp_tile(int rta) {
-    const int j0 = (16 * rta) / (NCU * NPF);
-    int j1 = (16 * rta + 15) / (NCU * NPF);
-    if (j1 > NFE - 1) j1 = NFE - 1;
-    int nn = (j1 - j0 + 1) * NPF;
-    if (nn > NPV) nn = NPV;
-    return (nn * NC + 3) / 4;
-  }
-  static constexpr int ksp() {
-    int m = 0;
-    for (int r = 0; r < TRH; ++r) m = ksp_tile(r) > m ? ksp_tile(r) : m;
-    return m;
-  }
-  static constexpr int KSP = ksp();
-  // GK's A fragments stay in registers across the RHS tiles when they fit
-  static constexpr bool GKREG = KSP * TRH <= 16;
-  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
-};
 
 // does k-step s of B (columns 4s..4s+3, column K = a + NPV*m + NPV*NCU*d)
 // meet a nonzero block of M^-1 C in D' column tile ct, or the du column?
@@ -139,10 +103,13 @@ struct S64Lds {
   unsigned char sTab[(NTAB + 7) / 8 * 8];        // klen[TRH] then the k-step table
 };
 
-// Element e of A (one 64-lane wavefront), LDS in L.
-template <class C>
+// Element e of A (one 64-lane wavefront), LDS in L.  TILE: F, GK and H are
+// in the fragment order of WsLayout (hdgx_shapes.h) and A.aek is the master's
+// AE table; otherwise every array has the reference layout.
+template <class C, bool TILE = false>
 HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   using S = S64Shape<C>;
+  using WL = WsLayout<C>;
   using LD = S64Lds<C>;
   static_assert(S::USE, "schur64: one to three row tiles, NU a multiple of 4 below the padded size");
   constexpr int NU = S::NU, NH = S::NH, NCOL = S::NCOL, NPV = S::NPV, NCU = S::NCU, ND = S::ND;
@@ -157,14 +124,14 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   const int lane = threadIdx.x;
   const int g = lane >> 4, c = lane & 15;
   if (e >= A.ne) return;
-  const bool sparse = A.aek != nullptr;
+  const bool sparse = TILE || A.aek != nullptr;
   double* const sX = L.sX;
   int* const sStep = L.sStep;
   int st = 0;
   const double* BDe = A.BD + e * (int64_t)NU * NCOL;
-  const double* Fe = A.F + e * (int64_t)NU * NH;
-  const double* GKe = A.GK + e * (int64_t)NH * NCOL;
-  const double* He = A.H + e * (int64_t)NH * NH;
+  const double* Fe = A.F + e * (int64_t)WL::f(TILE);
+  const double* GKe = A.GK + e * (int64_t)WL::gk(TILE);
+  const double* He = A.H + e * (int64_t)WL::h(TILE);
   const int LDO = NPV * A.nco;                   // DUDG / DUDG_DUH rows kept
 
   // ---- D' and du's right-hand side in column DU, loaded first so that their
@@ -258,7 +225,12 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = 16 * rt + 4 * i + g, j = 16 * tb + c;
-        acc[rt][RT + tb][i] = (row < NU && j < NH) ? -Fe[row + NU * s64_t<C>(j)] : 0.0;
+        if constexpr (TILE) {
+          const int ri = 4 * rt + i;
+          acc[rt][RT + tb][i] = ri < NU / 4 ? -Fe[(ri * TRH + tb) * 64 + lane] : 0.0;
+        } else {
+          acc[rt][RT + tb][i] = (row < NU && j < NH) ? -Fe[row + NU * s64_t<C>(j)] : 0.0;
+        }
       }
   // ---- D' += B M^-1 C, du's column -= B M^-1 Rq, RHS += B M^-1 E, over
   // k-steps of B's columns whose blocks are nonzero (compile-time masks), A
@@ -429,18 +401,40 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
     if (!isfinite(v)) st |= 4;
     A.dudg[e * LDO + r] = v;
   }
-  for (int row = lane; row < NH; row += 64) {
-    const int rta = row >> 4;
-    const int klen = L.sTab[rta];
-    const unsigned char* kt = L.sTab + TRH + rta * 4 * KSMAX;
-    double v = 0.0;
-    for (int idx = 0; idx < klen; ++idx) {
-      const int k = kt[idx];
-      v += GKe[row + NH * k] * sX[k];
+  if constexpr (TILE) {
+    // lane (g, c) sums row 16 rta + c over the k-steps' lane group g, then the
+    // four lane groups are added
+#pragma unroll
+    for (int rta = 0; rta < TRH; ++rta) {
+      const int kl = (__builtin_amdgcn_readfirstlane(L.sTab[rta]) + 3) / 4;
+      double v = 0.0;
+      for (int s = 0; s < kl; ++s) {
+        const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
+        if (k != AE_NONE) v += GKe[(s * TRH + rta) * 64 + lane] * sX[k];
+      }
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      const int row = 16 * rta + c;
+      if (g == 0 && row < NH) {
+        v = A.Rh[e * NH + row] - v;
+        if (!isfinite(v)) st |= 4;
+        A.fe[e * NH + row] = v;
+      }
     }
-    v = A.Rh[e * NH + row] - v;
-    if (!isfinite(v)) st |= 4;
-    A.fe[e * NH + row] = v;
+  } else {
+    for (int row = lane; row < NH; row += 64) {
+      const int rta = row >> 4;
+      const int klen = L.sTab[rta];
+      const unsigned char* kt = L.sTab + TRH + rta * 4 * KSMAX;
+      double v = 0.0;
+      for (int idx = 0; idx < klen; ++idx) {
+        const int k = kt[idx];
+        v += GKe[row + NH * k] * sX[k];
+      }
+      v = A.Rh[e * NH + row] - v;
+      if (!isfinite(v)) st |= 4;
+      A.fe[e * NH + row] = v;
+    }
   }
 
   // ---- A fragments of MiC for the q-row products (row n = l%16, k a = 4s + l/16)
@@ -469,7 +463,8 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
         for (int rta = 0; rta < TRH; ++rta) {
           const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
           const int row = 16 * rta + c;
-          gkf[s][rta] = (s < kend[rta] && k != AE_NONE && row < NH) ? GKe[row + NH * k] : 0.0;
+          if constexpr (TILE) gkf[s][rta] = s < kend[rta] ? GKe[(s * TRH + rta) * 64 + lane] : 0.0;
+          else gkf[s][rta] = (s < kend[rta] && k != AE_NONE && row < NH) ? GKe[row + NH * k] : 0.0;
         }
     }
   }
@@ -495,8 +490,12 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
       for (int i = 0; i < 4; ++i) {
         const int row = 16 * rta + 4 * i + g;
         double v = 0.0;
-        if (row < NH && j < NH && (!sparse || row / (NCU * NPF) == (j % NDF) / NPF))
+        if constexpr (TILE) {
+          const int rh = 4 * rta + i;
+          if (rh < NH / 4) v = He[(tb * (NH / 4) + rh) * 64 + lane];
+        } else if (row < NH && j < NH && (!sparse || row / (NCU * NPF) == (j % NDF) / NPF)) {
           v = He[row + NH * s64_t<C>(j)];
+        }
         ae[rta][i] = v;
       }
     __syncthreads();
@@ -567,7 +566,8 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
             const int k = s < KSMAX ? L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g] : AE_NONE;
             const int row = 16 * rta + c;
             const bool ok = s < kend[rta] && k != AE_NONE;
-            a[sb][rta] = (ok && row < NH) ? GKe[row + NH * k] : 0.0;
+            if constexpr (TILE) a[sb][rta] = (s < kend[rta] && s < S::KSP) ? GKe[(s * TRH + rta) * 64 + lane] : 0.0;
+            else a[sb][rta] = (ok && row < NH) ? GKe[row + NH * k] : 0.0;
             b[sb][rta] = ok ? sX[k + NCOLP * c] : 0.0;
           }
 #pragma unroll
@@ -603,10 +603,10 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   if (lane == 0 && A.status) A.status[e] = st;
 }
 
-template <class C>
+template <class C, bool TILE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S64Shape<C>::WPE, S64Shape<C>::WPE))) void schur64_kernel(SchurArgs A) {
   __shared__ S64Lds<C> lds;
-  schur64_body<C>(A, blockIdx.x, lds);
+  schur64_body<C, TILE>(A, blockIdx.x, lds);
 }
 
 }  // namespace hdgx

@@ -6,6 +6,7 @@
 
 #include "hdgx_device.h"
 #include "hdgx_kernels.h"
+#include "hdgx_shapes.h"
 
 namespace hdgx {
 
@@ -49,10 +50,14 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
   A.udg = sg.udg + e * NPV * NC;
   A.uh = sg.uh + e * NH;
   A.sh = sg.sh ? sg.sh + e * NPV * NC : nullptr;
-  A.BD = S.BD + w * NU * NCOL;
-  A.F = S.F + w * NU * NH;
-  A.GK = S.GK + w * NH * NCOL;
-  A.H = S.H + w * NH * NH;
+  // the workspace of a batched launch has the tile layout where the
+  // elimination takes it (WsLayout<C>::TILE_OK)
+  constexpr bool TILE = WsLayout<C>::TILE_OK;
+  using L = WsLayout<C>;
+  A.BD = S.BD + w * L::bd;
+  A.F = S.F + w * L::f(TILE);
+  A.GK = S.GK + w * L::gk(TILE);
+  A.H = S.H + w * L::h(TILE);
   A.Ru = S.Ru + w * NU;
   A.Rh = S.Rh + w * NH;
   if constexpr (C::UQ) {
@@ -65,7 +70,7 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
   }
   A.status = nullptr;
   __shared__ QuadLds<C> lds;
-  quad_body<C>(A, T, ap, 0, lds);
+  quad_body<C, false, TILE>(A, T, ap, 0, lds);
 }
 
 // the elimination arguments of batch element g (workspace slot w) as a
@@ -78,10 +83,12 @@ HDGX_DEV SchurArgs seg_schur_args(const SegTable& S, int64_t w) {
   const int64_t e = g - sg.e0;
   SchurArgs A;
   A.ne = 1;
-  A.BD = S.BD + w * NU * NCOL;
-  A.F = S.F + w * NU * NH;
-  A.GK = S.GK + w * NH * NCOL;
-  A.H = S.H + w * NH * NH;
+  constexpr bool TILE = WsLayout<C>::TILE_OK;
+  using L = WsLayout<C>;
+  A.BD = S.BD + w * L::bd;
+  A.F = S.F + w * L::f(TILE);
+  A.GK = S.GK + w * L::gk(TILE);
+  A.H = S.H + w * L::h(TILE);
   A.Ru = S.Ru + w * NU;
   A.Rh = S.Rh + w * NH;
   if constexpr (C::UQ) {
@@ -115,7 +122,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>()
   if constexpr (S64Shape<C>::USE) {
     A.aek = aek;
     __shared__ S64Lds<C> lds;
-    schur64_body<C>(A, 0, lds);
+    schur64_body<C, true>(A, 0, lds);
   } else {
     __shared__ SchurLds<C> lds;
     schur_mfma_body<C>(A, perm, 0, lds);
