@@ -870,6 +870,38 @@ int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg, con
   return HDGX_OK;
 }
 
+int hdgx_elemental_batch_mixed(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
+                               const hdgx_segment* segs) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (nseg < 0 || (nseg > 0 && (!segs || !masters))) return fail(ctx, HDGX_ERR_ARG, "batch_mixed: arguments");
+  if (nseg == 0) return HDGX_OK;
+  // segments grouped by their master's order, each group in its given order
+  std::vector<int> order(nseg);
+  std::vector<int> pof(nseg);
+  for (int s = 0; s < nseg; ++s) {
+    if (!masters[s]) return fail(ctx, HDGX_ERR_ARG, "batch_mixed: null master");
+    if (masters[s]->nd != masters[0]->nd) return fail(ctx, HDGX_ERR_ARG, "batch_mixed: segments must share nd");
+    pof[s] = porder_of(masters[s]->nd, masters[s]->npv);
+    if (pof[s] < 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "batch_mixed: order");
+    order[s] = s;
+  }
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return pof[x] > pof[y]; });
+  std::vector<hdgx_segment> group;
+  for (int k = 0; k < nseg;) {
+    const int p = pof[order[k]];
+    const hdgx_master* m = masters[order[k]];
+    group.clear();
+    for (; k < nseg && pof[order[k]] == p; ++k) {
+      if (masters[order[k]]->npv != m->npv || masters[order[k]]->npf != m->npf)
+        return fail(ctx, HDGX_ERR_ARG, "batch_mixed: masters of one order differ");
+      group.push_back(segs[order[k]]);
+    }
+    const int rc = hdgx_elemental_batch(ctx, m, (int)group.size(), group.data());
+    if (rc) return rc;
+  }
+  return HDGX_OK;
+}
+
 }  // extern "C"
 
 // ------------------------------------------------------------ launch plans

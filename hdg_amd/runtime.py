@@ -54,6 +54,7 @@ def lib():
         L.hdgx_schur_u.argtypes = [vp, i32, i32, i32, i32, i64] + [vp] * 11
         L.hdgx_schur_uq.argtypes = [vp, i32, i32, i32, i32, i32, i64] + [vp] * 15
         L.hdgx_elemental_batch.argtypes = [vp, vp, i32, vp]
+        L.hdgx_elemental_batch_mixed.argtypes = [vp, i32, vp, vp]
         L.hdgx_plan_create.argtypes = [vp, i32, vp, vp, i32, i32, ctypes.POINTER(vp)]
         L.hdgx_plan_launch.argtypes = [vp, vp]
         L.hdgx_plan_capture.argtypes = [vp, vp]
@@ -79,7 +80,7 @@ def lib():
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags",
            "hdgx_set_workspace_limit", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
-           "hdgx_elemental_batch", "hdgx_elemental_noalloc",
+           "hdgx_elemental_batch", "hdgx_elemental_batch_mixed", "hdgx_elemental_noalloc",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2",
@@ -275,6 +276,17 @@ class Context:
                               dudg_duh.ptr.value, st.ptr.value if st is not None else None,
                               ctypes.cast(ctypes.pointer(ca), ctypes.c_void_p).value)
         self._check(lib().hdgx_elemental_batch(self.h, ctypes.byref(mc.c), len(segments), arr))
+
+    def elemental_batch_mixed(self, masters, segments):
+        """hdgx_elemental_batch_mixed: segments of several orders in one call,
+        masters[k] the master of segments[k] (segments as elemental_batch)."""
+        if not segments:
+            return
+        mcs = [self.master_c(m) for m in masters]
+        mp = (ctypes.c_void_p * len(masters))(*[ctypes.cast(ctypes.pointer(mc.c), ctypes.c_void_p).value
+                                                for mc in mcs])
+        arr, keep = self._segments(segments)
+        self._check(lib().hdgx_elemental_batch_mixed(self.h, len(segments), mp, arr))
 
     def elemental_matrices(self, master, app, dgnodes, bf, udg, uh, sh=None, out=None):
         """The element matrices as volint + faceint hand them to schur_primal

@@ -7,9 +7,14 @@ every admitted request's state in HBM and advances all of them together:
 
   * admission: a request that has arrived is admitted as soon as its HBM
     footprint fits the budget and a slot is free;
-  * elemental phase: the requests waiting for an elemental evaluation are
-    grouped by order p and each group runs as ONE batched launch
-    (hdgx_elemental_batch), every request with its own physics parameters;
+  * admission builds the request's mesh connectivity and initial trace on
+    the device (hdg_amd.devmesh: hdgx_mesh_faces / _elcon / _f2f), on the
+    clock;
+  * elemental phase, one call in flight at a time: the requests that became
+    ready while the previous call ran, of every order p, run as ONE
+    mixed-order batched call
+    (hdgx_elemental_batch_mixed: a batched launch per order present, back to
+    back), every request with its own physics parameters;
   * trace phase: each request's face-block system, block ILU(0), GMRES and
     local update run on a worker stream (hdgx_trace_*, hdgx_gmres,
     hdgx_local_update), so the trace solves of several requests and the next
@@ -75,11 +80,15 @@ def make_workload(nreq=512, seed=2024, rate=50.0, ne_lo=500, ne_hi=50000, levels
 
 
 class PayloadFactory:
-    """Host inputs of a request (the request's message): mesh and trace
-    connectivity cached per (n, p), state from the request's seed."""
+    """Request messages: what a client sends -- the mesh geometry (vertices,
+    triangles and DG node coordinates, cached per (n, p) since the workload
+    reuses 16 mesh sizes), the physics parameters and the state from the
+    request's seed.  The connectivity (faces, trace numbering, face tables,
+    boundary tags) and the initial trace are the server's work, built on the
+    device on admission (hdg_amd.devmesh)."""
 
     def __init__(self):
-        self._mesh = {}
+        self._geom = {}
         self._master = {}
 
     def master(self, p):
@@ -88,38 +97,58 @@ class PayloadFactory:
             self._master[p] = mkmaster(2, p)
         return self._master[p]
 
-    def mesh(self, n, p, physics):
+    def geometry(self, n, p):
+        """(vertices (nv, 2), t (ne, 3), dgnodes (npv, 2, ne)) of the jittered
+        2 n^2-triangle square (mesh/squaremesh.m nodes, mesh/mkmesh/createnodes.m)."""
+        from .mesh import createnodes, jitter, squaremesh
+        key = (n, p)
+        if key not in self._geom:
+            vert, t = squaremesh(n + 1, n + 1)
+            vert = jitter(vert, 1.0 / n, 0.1, seed=1000 + n)
+            self._geom[key] = (vert, t, createnodes(vert, t, self.master(p).plocvl))
+        return self._geom[key]
+
+    # the host-built mesh of the same geometry (hdg_amd/mesh.py), for checks
+    def host_mesh(self, n, p):
         from .mesh import mkmesh_square
-        from .trace import color_faces, face_connectivity, mkf2f
-        key = (n, p, physics)
-        if key not in self._mesh:
-            m = self.master(p)
-            mesh = mkmesh_square(n, n, m, 0.1, seed=1000 + n, ncd_extra=0)
-            f2f = mkf2f(mesh)
-            conn = dict(fconn=face_connectivity(mesh), f2f=f2f, color=color_faces(f2f))
-            self._mesh[key] = (mesh, conn)
-        return self._mesh[key]
+        return mkmesh_square(n, n, self.master(p), 0.1, seed=1000 + n, ncd_extra=0)
 
     def build(self, spec):
         from .app import euler_app, ns_app
-        from .problems import euler_state, inituhat, ns_state
+        from .problems import euler_state, ns_state
         m = self.master(spec.p)
-        mesh, conn = self.mesh(spec.n, spec.p, spec.physics)
+        vert, t, dgnodes = self.geometry(spec.n, spec.p)
         if spec.physics == PHYS_NS:
             app = ns_app(minf=spec.minf)
-            udg = ns_state(mesh.dgnodes, 1.4, spec.minf, spec.seed)
+            udg = ns_state(dgnodes, 1.4, spec.minf, spec.seed)
         else:
             app = euler_app(2, minf=spec.minf)
-            udg = euler_state(mesh.dgnodes, 2, 1.4, spec.minf, spec.seed)
-        UH = inituhat(m, mesh.elcon, udg, app.ncu, mesh.nsiz)
+            udg = euler_state(dgnodes, 2, 1.4, spec.minf, spec.seed)
         sh = np.zeros_like(udg)
         if spec.dt > 0:
             # BDF1 (hdg_solve_bdf.m:29-43 with torder 1): fc_u = 1/dt, SH = u_n/dt
             app.tdep = 1
             app.fc_u = 1.0 / spec.dt
             sh[:, :app.ncu, :] = udg[:, :app.ncu, :] / spec.dt
-        return dict(master=m, app=app, mesh=mesh, conn=conn, udg=np.asfortranarray(udg),
-                    UH=np.asfortranarray(UH), sh=np.asfortranarray(sh))
+        return dict(master=m, app=app, vertices=vert, t=t, dgnodes=dgnodes, udg=np.asfortranarray(udg),
+                    sh=np.asfortranarray(sh), dt=spec.dt)
+
+
+def faces_of(spec):
+    """Faces of the request's 2 n^2-triangle square: 3 n^2 + 2 n."""
+    return 3 * spec.n * spec.n + 2 * spec.n
+
+
+def element_matrix_bytes(master, app):
+    """Bytes of one element's matrices in the elemental call's workspace (the
+    reference layout: BD, F, GK, H, Ru, Rh and, with q, M, C, E, Rq)."""
+    npv, ncu, nc, nd = master.npv, app.ncu, app.nc, master.nd
+    ndf = master.npf * master.nfe
+    nu, nh, ncol = npv * ncu, ncu * ndf, npv * nc
+    w = nu * ncol + nu * nh + nh * ncol + nh * nh + nu + nh
+    if app.localsolve:
+        w += npv * npv + npv * npv * nd + npv * ndf * nd + npv * ncu * nd
+    return 8 * w
 
 
 def footprint_bytes(spec, master, app, nf):
@@ -148,24 +177,36 @@ class Slot:
     gmres_iters: list = field(default_factory=list)
     views: tuple = None
     event: object = None
+    admit_s: float = 0.0
 
 
 class _ReqState:
-    """Device arrays of an admitted request (the elemental outputs are views of
-    a batch launch's buffers)."""
+    """Device arrays of an admitted request; its connectivity and initial
+    trace are built on the device (hdg_amd.devmesh) with the mesh context
+    `mctx`.  The elemental outputs are views of a batch launch's buffers."""
 
-    def __init__(self, ctx, payload, stream):
+    def __init__(self, mctx, payload, stream):
+        from . import devmesh
         torch = _torch()
-        m, app, mesh = payload["master"], payload["app"], payload["mesh"]
-        conn = payload["conn"]
+        m, app = payload["master"], payload["app"]
+        self.mesh, self.UH, self.ts = devmesh.build(mctx, payload["vertices"], payload["t"], m,
+                                                    payload["dgnodes"], payload["udg"], app.ncu)
+        mesh = self.mesh
         with torch.cuda.stream(stream):
-            self.ts = TraceSystem(ctx, mesh, app.ncu, with_bilu0=True, conn=conn)
-            self.dgnodes = Dev.from_numpy(mesh.dgnodes)
-            self.bf = Dev.from_numpy(mesh.bf, np.int32)
-            self.UDG = Dev.from_numpy(payload["udg"])
-            self.UH = Dev.from_numpy(payload["UH"])
-            self.SH = Dev.from_numpy(payload["sh"])
             ts = self.ts
+            self.dgnodes = mesh.dgnodes
+            self.bf = mesh.bf
+            self.UDG = Dev.from_numpy(payload["udg"])
+            if app.tdep:
+                # BDF1 source SH = u_n / dt of the state just sent (hdg_solve_bdf.m:29-43,
+                # torder 1), formed on the device instead of sent: zero q columns
+                npv, nc, ne = self.UDG.shape
+                U = self.UDG.t.view(ne, nc, npv)
+                sh = torch.zeros_like(U)
+                sh[:, :app.ncu, :] = U[:, :app.ncu, :] / payload["dt"]
+                self.SH = Dev(sh.flatten(), (npv, nc, ne))
+            else:
+                self.SH = Dev.from_numpy(payload["sh"])
             self.uhe = Dev.empty((app.ncu, m.npf * m.nfe, mesh.ne))
             self.K = Dev.empty((ts.ncf, ts.ncf, ts.nbf - 1, ts.nf))
             self.F = Dev.empty((ts.ncf, ts.nf))
@@ -183,21 +224,51 @@ class Server:
     its own hdgx context and stream)."""
 
     def __init__(self, device=0, budget_bytes=None, slots=64, workers=4, restart=20, gmres_tol=1e-6,
-                 max_batch_elems=2_000_000):
+                 max_batch_elems=2_000_000, admitters=2):
         torch = _torch()
         self.device = device
         if budget_bytes is None:
             free, total = torch.cuda.mem_get_info(device)
             budget_bytes = int(0.6 * free)
+            # the elemental context's workspace (24 GiB at most) comes out of it
+            from .master import mkmaster
+            from .app import ns_app
+            per = element_matrix_bytes(mkmaster(2, 4), ns_app())
+            budget_bytes -= min(24 << 30, max_batch_elems * per)
         self.budget = budget_bytes
         self.slots = slots
         self.nworkers = workers
+        self.nadmit = admitters
         self.restart, self.gmres_tol = restart, gmres_tol
         self.max_batch_elems = max_batch_elems
         self.ctx = Context(device)
         self.main_stream = torch.cuda.current_stream(device)
 
+
     # ------------------------------------------------------------- workers
+    def _admitter(self, aq, doneq):
+        """Admission: a request's connectivity, initial trace and state on the
+        device, on a context and stream of its own (the mesh entry points
+        synchronise their stream, which must not wait for queued batches)."""
+        torch = _torch()
+        stream = torch.cuda.Stream(self.device)
+        ctx = Context(self.device)
+        ctx.bind_stream(stream)
+        while True:
+            slot = aq.get()
+            if slot is None:
+                return
+            try:
+                t = time.perf_counter()
+                with torch.cuda.stream(stream):
+                    slot.state = _ReqState(ctx, slot.payload, stream)
+                stream.synchronize()
+                slot.admit_s = time.perf_counter() - t
+            except Exception as exc:
+                doneq.put(("built", slot, exc))
+                continue
+            doneq.put(("built", slot, None))
+
     def _worker(self, wq, doneq):
         torch = _torch()
         stream = torch.cuda.Stream(self.device)
@@ -214,9 +285,9 @@ class Server:
                     self._trace_step(ctx, slot)
                     stream.synchronize()
             except Exception as exc:      # surfaced by the scheduler
-                doneq.put((slot, exc))
+                doneq.put(("done", slot, exc))
                 continue
-            doneq.put((slot, None))
+            doneq.put(("done", slot, None))
 
     def _trace_step(self, ctx, slot):
         S = slot.state
@@ -236,14 +307,14 @@ class Server:
 
     # ------------------------------------------------------------ elemental
     def _elemental_batches(self, ready):
-        """Group ready slots by (physics, p) and launch one batch per group."""
-        torch = _torch()
+        """Ready requests of one physics, whatever their orders, go into one
+        mixed-order batched call (hdgx_elemental_batch_mixed); very large sets
+        are split so that one call's output buffers stay bounded."""
         groups = {}
         for s in ready:
-            groups.setdefault((s.spec.physics, s.spec.p), []).append(s)
+            groups.setdefault(s.spec.physics, []).append(s)
         launched = []
-        for (phys, p), slots in groups.items():
-            # split very large groups so one batch's buffers stay bounded
+        for phys, slots in groups.items():
             chunk, cur = [], 0
             for s in slots:
                 if chunk and cur + s.spec.ne > self.max_batch_elems:
@@ -257,38 +328,42 @@ class Server:
 
     def _launch_group(self, slots):
         torch = _torch()
-        m = slots[0].payload["master"]
-        app0 = slots[0].payload["app"]
-        npv, ncu, nc = m.npv, app0.ncu, app0.nc
-        ndf = m.npf * m.nfe
-        nh = ncu * ndf
-        NE = sum(s.spec.ne for s in slots)
-        AE = torch.empty(nh * nh * NE, dtype=torch.float64, device="cuda")
-        FE = torch.empty(nh * NE, dtype=torch.float64, device="cuda")
-        DU = torch.empty(npv * nc * NE, dtype=torch.float64, device="cuda")
-        DX = torch.empty(npv * nc * nh * NE, dtype=torch.float64, device="cuda")
-        ST = torch.empty(NE, dtype=torch.int32, device="cuda")
-        segs = []
-        e0 = 0
+        sizes = []
         for s in slots:
-            S = s.state
+            m, app = s.payload["master"], s.payload["app"]
+            ndf = m.npf * m.nfe
+            nh = app.ncu * ndf
             ne = s.spec.ne
+            sizes.append((nh * nh * ne, nh * ne, m.npv * app.nc * ne, m.npv * app.nc * nh * ne, ne))
+        tot = [sum(z[k] for z in sizes) for k in range(5)]
+        AE = torch.empty(tot[0], dtype=torch.float64, device="cuda")
+        FE = torch.empty(tot[1], dtype=torch.float64, device="cuda")
+        DU = torch.empty(tot[2], dtype=torch.float64, device="cuda")
+        DX = torch.empty(tot[3], dtype=torch.float64, device="cuda")
+        ST = torch.empty(tot[4], dtype=torch.int32, device="cuda")
+        segs, masters = [], []
+        off = [0] * 5
+        for s, z in zip(slots, sizes):
+            S = s.state
+            m, app = s.payload["master"], s.payload["app"]
+            npv, ncu, nc, ndf, ne = m.npv, app.ncu, app.nc, m.npf * m.nfe, s.spec.ne
             S.ts.ctx = self.ctx
             S.ts.gather(S.UH, S.uhe)
-            v = (Dev(AE[nh * nh * e0:nh * nh * (e0 + ne)], (ncu, ndf, ncu, ndf, ne)),
-                 Dev(FE[nh * e0:nh * (e0 + ne)], (ncu, ndf, ne)),
-                 Dev(DU[npv * nc * e0:npv * nc * (e0 + ne)], (npv, nc, ne)),
-                 Dev(DX[npv * nc * nh * e0:npv * nc * nh * (e0 + ne)], (npv, nc, ncu, ndf, ne)),
-                 Dev(ST[e0:e0 + ne], (ne,)))
+            v = (Dev(AE[off[0]:off[0] + z[0]], (ncu, ndf, ncu, ndf, ne)),
+                 Dev(FE[off[1]:off[1] + z[1]], (ncu, ndf, ne)),
+                 Dev(DU[off[2]:off[2] + z[2]], (npv, nc, ne)),
+                 Dev(DX[off[3]:off[3] + z[3]], (npv, nc, ncu, ndf, ne)),
+                 Dev(ST[off[4]:off[4] + z[4]], (ne,)))
+            off = [a + b for a, b in zip(off, z)]
             s.views = v
-            segs.append((s.payload["app"], S.dgnodes, S.bf, S.UDG, S.uhe, S.SH, v))
-            e0 += ne
-        self.ctx.elemental_batch(m, segs)
+            segs.append((app, S.dgnodes, S.bf, S.UDG, S.uhe, S.SH, v))
+            masters.append(m)
+        self.ctx.elemental_batch_mixed(masters, segs)
         ev = torch.cuda.Event()
         ev.record(self.main_stream)
         for s in slots:
             s.event = ev
-        return len(slots), NE
+        return len(slots), tot[4], len({s.spec.p for s in slots})
 
     # ----------------------------------------------------------------- loop
     def run(self, specs, factory=None, realtime=True, log=None, keep_state=False):
@@ -299,9 +374,11 @@ class Server:
         factory = factory or PayloadFactory()
         pending = sorted(specs, key=lambda s: s.t_arrival)
         payloads = {s.rid: factory.build(s) for s in pending}      # request messages, prepared off the clock
-        wq, doneq = queue.Queue(), queue.Queue()
+        wq, aq, doneq = queue.Queue(), queue.Queue(), queue.Queue()
         threads = [threading.Thread(target=self._worker, args=(wq, doneq), daemon=True) for _ in range(self.nworkers)]
-        for t in threads:
+        admitters = [threading.Thread(target=self._admitter, args=(aq, doneq), daemon=True)
+                     for _ in range(self.nadmit)]
+        for t in threads + admitters:
             t.start()
         used = 0
         active = []
@@ -309,50 +386,65 @@ class Server:
         finished = []
         nbatches = 0
         batched_elems = 0
+        admit_s = 0.0           # device mesh builds and uploads on admission
+        mixed_batches = 0       # elemental calls over requests of several orders
+        last_call = None        # device event of the last elemental call
         t0 = time.perf_counter()
         clock = (lambda: time.perf_counter() - t0) if realtime else (lambda: float("inf"))
-        inflight = 0
+        inflight = 0            # admissions and trace steps not reported yet
         try:
             while pending or active:
                 now = clock()
-                # admission by arrival, slots and HBM budget
+                # admission by arrival, slots and HBM budget; the device work of
+                # an admission runs on an admission thread
                 while pending and pending[0].t_arrival <= now and len(active) < self.slots:
                     spec = pending[0]
                     pl = payloads[spec.rid]
-                    b = footprint_bytes(spec, pl["master"], pl["app"], pl["mesh"].nf)
+                    b = footprint_bytes(spec, pl["master"], pl["app"], faces_of(spec))
                     if used + b > self.budget and active:
                         break
                     pending.pop(0)
                     slot = Slot(spec, pl, b)
                     slot.t_admit = time.perf_counter() - t0
-                    slot.state = _ReqState(self.ctx, pl, self.main_stream)
                     used += b
                     active.append(slot)
-                    ready.append(slot)
-                if ready:
-                    for n, ne in self._elemental_batches(ready):
+                    aq.put(slot)
+                    inflight += 1
+                # one elemental call at a time (iteration-level batching): what
+                # becomes ready while the previous call still runs on the
+                # device joins the next one instead of queueing behind it alone
+                if ready and (last_call is None or last_call.query()):
+                    for n, ne, norders in self._elemental_batches(ready):
                         nbatches += 1
                         batched_elems += ne
+                        mixed_batches += norders > 1
+                    last_call = ready[-1].event
                     for s in ready:
                         wq.put(s)
                         inflight += 1
                     ready = []
-                # collect finished trace steps; wait when nothing else can progress
+                # collect admissions and finished trace steps; wait when nothing
+                # else can progress
                 timeout = None
                 if pending and len(active) < self.slots:
                     timeout = max(0.0, pending[0].t_arrival - clock()) if realtime else 0.0
                 if inflight == 0 and timeout is None:
                     timeout = 0.0
+                if ready:                   # the previous call is still running: poll
+                    timeout = 2e-4 if timeout is None else min(timeout, 2e-4)
                 try:
                     item = doneq.get(timeout=timeout) if (timeout is None or timeout > 0) else doneq.get_nowait()
                 except queue.Empty:
                     continue
                 while True:
-                    slot, exc = item
+                    kind, slot, exc = item
                     inflight -= 1
                     if exc is not None:
                         raise exc
-                    if slot.it < slot.spec.iters:
+                    if kind == "built":
+                        admit_s += slot.admit_s
+                        ready.append(slot)
+                    elif slot.it < slot.spec.iters:
                         ready.append(slot)
                     else:
                         slot.t_done = time.perf_counter() - t0
@@ -374,7 +466,9 @@ class Server:
         finally:
             for _ in threads:
                 wq.put(None)
-            for t in threads:
+            for _ in admitters:
+                aq.put(None)
+            for t in threads + admitters:
                 t.join()
         makespan = time.perf_counter() - t0
         lat = np.array([(s.t_done - (s.spec.t_arrival if realtime else 0.0)) for s in finished])
@@ -390,6 +484,8 @@ class Server:
             "elemental_batches": nbatches,
             "mean_requests_per_batch": (sum(s.spec.iters for s in finished) / nbatches) if nbatches else None,
             "gmres_iters_mean": float(np.mean([np.mean(s.gmres_iters) for s in finished])) if finished else None,
+            "admission_ms_mean": admit_s / len(finished) * 1e3 if finished else None,
+            "mixed_order_batches": mixed_batches,
         }
         return stats, finished
 

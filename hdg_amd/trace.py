@@ -131,6 +131,35 @@ class TraceSystem:
             c.face_rank = self.face_rank.ptr.value
         self.c = c
 
+    @classmethod
+    def from_device(cls, ctx, ncu, npf, nfe, ne, nf, fconn, f2f, f2f_host, elcon, with_bilu0=True):
+        """The same from connectivity already on the device (hdg_amd.devmesh):
+        fconn (4, nf), f2f (2nfe-1, nf), elcon (npf*nfe, ne) Dev int32 arrays and
+        f2f_host, the face table on the host for the face colouring."""
+        self = cls.__new__(cls)
+        self.ctx = ctx
+        self.mesh = None
+        self.ncu, self.npf, self.nfe = ncu, npf, nfe
+        self.ne, self.nf = ne, nf
+        self.ncf = ncu * npf
+        self.nbf = 2 * nfe - 1
+        self.fconn, self.f2f, self.elcon = fconn, f2f, elcon
+        self.f2f_host = f2f_host
+        c = CTrace(ncu, npf, nfe, ne, nf, fconn.ptr.value, f2f.ptr.value, elcon.ptr.value, 0, None, None, None)
+        if with_bilu0:
+            ptr, rows, rank, color = color_faces(f2f_host)
+            self.ncolors = len(ptr) - 1
+            self._color_ptr = (ctypes.c_int32 * len(ptr))(*ptr.tolist())
+            self.color_rows = Dev.from_numpy(rows, np.int32)
+            self.face_rank = Dev.from_numpy(rank, np.int32)
+            self.face_color = color
+            c.ncolors = self.ncolors
+            c.color_ptr = ctypes.cast(self._color_ptr, ctypes.c_void_p)
+            c.color_rows = self.color_rows.ptr.value
+            c.face_rank = self.face_rank.ptr.value
+        self.c = c
+        return self
+
     # ---------------------------------------------------------------- operators
     def _check(self, rc):
         self.ctx._check(rc)

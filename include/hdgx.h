@@ -253,6 +253,17 @@ typedef struct hdgx_segment {
 int hdgx_elemental_batch(hdgx_ctx* ctx, const hdgx_master* master, int nseg,
                          const hdgx_segment* segs);
 
+/* The same over segments of several polynomial orders (one dimension):
+ * masters[s] is the master of segment s.  Segments are grouped by order and
+ * each group is one hdgx_elemental_batch (one batched launch per kernel),
+ * the groups back to back on the context's stream, highest order first;
+ * results equal those of hdgx_elemental_batch on each group.  For callers
+ * whose batches change every call (a serving loop) -- a fixed batch replayed
+ * many times is a launch plan (below).  No counterpart in the reference, whose
+ * element loop runs one mesh at a time (kernel/hdg_assemble.m:40-43). */
+int hdgx_elemental_batch_mixed(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
+                               const hdgx_segment* segs);
+
 /* Launch plan: a batch of segments of several polynomial orders (one physics
  * and dimension), with tables and element-matrix workspace set up once so
  * that the same batch can be launched many times, and captured into a

@@ -11,7 +11,7 @@ from tests.gpu_util import relerr
 
 pytestmark = [pytest.mark.gpu, pytest.mark.skipif(not has_gpu(), reason="needs a GPU")]
 
-from hdg_amd.problems import euler2d_square, gather_uh, ns2d_square, with_bdf1  # noqa: E402
+from hdg_amd.problems import euler2d_square, gather_uh, inituhat, ns2d_square, with_bdf1  # noqa: E402
 from oracle import trace_oracle as TO  # noqa: E402
 
 
@@ -66,7 +66,12 @@ def test_served_result_is_the_same_alone_and_in_the_mix():
     stats, done = srv.run(specs, factory=fac, realtime=False, keep_state=True)
     assert stats["requests"] == len(specs)
     mix = {s.spec.rid: (s.state.UDG.numpy(), s.state.UH.numpy()) for s in done}
-    assert stats["elemental_batches"] < sum(s.iters for s in specs)      # requests were merged
+    # the 8 requests, of orders 1..4, are admitted together; requests whose
+    # admission finishes together share an elemental call across orders
+    print(f"[gpu]   {stats['elemental_batches']} batches, {stats['mean_requests_per_batch']:.2f} requests each, "
+          f"{stats['mixed_order_batches']} over several orders", flush=True)
+    assert stats["elemental_batches"] < sum(s.iters for s in specs)
+    assert stats["mixed_order_batches"] >= 1
     for rid in (0, 3, 5):
         print(f"[gpu] serve alone {rid}", flush=True)
         _, one = Server(workers=1, slots=1).run([specs[rid]], factory=fac, realtime=False, keep_state=True)
@@ -98,16 +103,19 @@ def test_served_bdf1_step_matches_the_oracle(oracle, p):
     print(f"[gpu] served step vs oracle p={p}", flush=True)
     _, done = Server(workers=1, slots=1, gmres_tol=1e-13).run([spec], factory=fac, realtime=False, keep_state=True)
     u_gpu, uh_gpu = done[0].state.UDG.numpy(), done[0].state.UH.numpy()
-    m, app, mesh = pl["master"], pl["app"], pl["mesh"]
-    udg, UH = pl["udg"].copy(), pl["UH"].copy()
+    m, app = pl["master"], pl["app"]
+    mesh = fac.host_mesh(spec.n, spec.p)
+    udg = pl["udg"].copy()
+    UH = inituhat(m, mesh.elcon, udg, app.ncu, mesh.nsiz)
     uh = gather_uh(UH, mesh.elcon)
     AE, FE, DUDG, DUDG_DUH, st = oracle.elemental(m, app, mesh.dgnodes, mesh.bf, udg, uh, pl["sh"])
     A, b = TO.sparse_system(AE, FE, mesh.elcon, app.ncu, mesh.nsiz)
     DUH = spla.spsolve(A.tocsc(), b).reshape(app.ncu, -1, order="F")
     udg[:, :, :] += TO.local_update(DUDG, DUDG_DUH, DUH, mesh.elcon)
     UH = UH + DUH
-    assert relerr(uh_gpu.reshape(UH.shape, order="F"), UH) < 1e-9
-    assert relerr(u_gpu, udg) < 1e-9
+    e_uh, e_u = relerr(uh_gpu.reshape(UH.shape, order="F"), UH), relerr(u_gpu, udg)
+    print(f"[gpu]   UH {e_uh:.2e} UDG {e_u:.2e}", flush=True)
+    assert e_uh < 1e-12 and e_u < 1e-12
 
 
 def test_realtime_serving_reports_latency():
@@ -117,3 +125,56 @@ def test_realtime_serving_reports_latency():
     stats, done = Server(workers=2, slots=8).run(specs)
     assert stats["requests"] == 12
     assert 0 < stats["p50_ms"] <= stats["p99_ms"]
+
+
+@pytest.mark.parametrize("n,p", [(3, 1), (6, 2), (9, 3), (5, 4), (40, 3)])
+def test_device_connectivity_equals_host_construction(n, p):
+    """The serving loop's on-admission connectivity (hdg_amd.devmesh, on the
+    device) equals the host construction (hdg_amd/mesh.py, trace.py,
+    problems.inituhat): bf, elcon, fconn, f2f and the face colouring integer
+    for integer, the initial trace bit for bit."""
+    from hdg_amd import devmesh
+    from hdg_amd.runtime import Context
+    from hdg_amd.serve import PayloadFactory, RequestSpec
+    from hdg_amd.trace import color_faces, face_connectivity, mkf2f
+    fac = PayloadFactory()
+    pl = fac.build(RequestSpec(0, 0.0, n, p, 0.4, 77, 0.05, "ns"))
+    m, app = pl["master"], pl["app"]
+    print(f"[gpu] device connectivity n={n} p={p}", flush=True)
+    ctx = Context(0)
+    dm, UH, ts = devmesh.build(ctx, pl["vertices"], pl["t"], m, pl["dgnodes"], pl["udg"], app.ncu)
+    mesh = fac.host_mesh(n, p)
+    assert np.array_equal(dm.bf.numpy(), mesh.bf)
+    assert np.array_equal(dm.elcon.numpy(), mesh.elcon)
+    assert np.array_equal(dm.fconn.numpy(), face_connectivity(mesh))
+    f2f = mkf2f(mesh)
+    assert np.array_equal(dm.f2f.numpy(), f2f)
+    assert np.array_equal(ts.face_color, color_faces(f2f)[3])
+    assert np.array_equal(UH.numpy(), inituhat(m, mesh.elcon, pl["udg"], app.ncu, mesh.nsiz))
+
+
+def test_mixed_order_batch_equals_each_order_batch():
+    """hdgx_elemental_batch_mixed over NS requests of orders 1..4 (interleaved)
+    gives each request the bits of its own order's hdgx_elemental_batch."""
+    from hdg_amd.runtime import Dev
+    print("[gpu] mixed-order batch", flush=True)
+    probs = [with_bdf1(ns2d_square(n, p, seed=90 + p, minf=m, av_frac=0.3), 0.05, seed=p)
+             for n, p, m in ((3, 2, 0.3), (2, 4, 0.5), (4, 1, 0.6), (3, 3, 0.4), (2, 2, 0.25), (5, 1, 0.45))]
+
+    def arrays(P):
+        npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.npf * P.master.nfe, P.ne
+        dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+               Dev.from_numpy(P.uh), Dev.from_numpy(P.sh))
+        out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+               Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+        return (P.app, *dev, out)
+
+    from hdg_amd.runtime import Context
+    ctx = Context(0)
+    segs = [arrays(P) for P in probs]
+    ctx.elemental_batch_mixed([P.master for P in probs], segs)
+    for P, seg in zip(probs, segs):
+        one = arrays(P)
+        ctx.elemental_batch(P.master, [one])
+        for a, b in zip(seg[6], one[6]):
+            assert np.array_equal(a.numpy(), b.numpy()), P.name

@@ -229,4 +229,6 @@ def test_device_newton_matches_the_oracle_newton(ctx, oracle):
     big = r_ref > 1e-6 * r_ref[0]
     assert np.allclose(r_dev[big], r_ref[big], rtol=1e-6), (r_dev, r_ref)
     assert r_dev[-1] <= 1e-10 * r_dev[0]
-    assert relerr(uhg, H) < 1e-9 and relerr(ug, u) < 1e-9
+    e_h, e_u = relerr(uhg, H), relerr(ug, u)
+    print(f"[gpu]   converged state vs oracle: UH {e_h:.2e} UDG {e_u:.2e}", flush=True)
+    assert e_h < 1e-12 and e_u < 1e-12
