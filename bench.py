@@ -57,30 +57,19 @@ def time_steps(torch, stream, fn, steps, warmup):
 
 
 def kernel_times(torch, ctx, P, dev, out, reps):
-    """Per-kernel durations with events on the launch stream, each over `reps`
-    back-to-back launches so that the host's launch work overlaps the device's:
-    the quadrature kernel alone (hdgx_elemental_matrices into the same arrays
-    every time) and the elimination kernel as hdgx_elemental runs it (the step
-    minus the quadrature)."""
-    s = ctx.stream
-    mats = ctx.elemental_matrices(P.master, P.app, *dev)
+    """Per-kernel durations inside hdgx_elemental: the library brackets its
+    quadrature and elimination launches with HIP events on the launch stream
+    (hdgx_set_timing), over `reps` calls."""
     ctx.elemental(P.master, P.app, *dev, out=out)
     torch.cuda.synchronize()
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
-    e[0].record(s)
-    for _ in range(reps):
-        ctx.elemental_matrices(P.master, P.app, *dev, out=mats)
-    e[1].record(s)
-    torch.cuda.synchronize()
-    del mats
-    e[2].record(s)
-    for _ in range(reps):
-        ctx.elemental(P.master, P.app, *dev, out=out)
-    e[3].record(s)
-    torch.cuda.synchronize()
-    tq = e[0].elapsed_time(e[1]) / reps
-    tt = e[2].elapsed_time(e[3]) / reps
-    return tq, tt - tq
+    ctx.set_timing(True)
+    try:
+        for _ in range(reps):
+            ctx.elemental(P.master, P.app, *dev, out=out)
+        tq, ts, n = ctx.kernel_times()
+    finally:
+        ctx.set_timing(False)
+    return tq / n, ts / n
 
 
 def measured_traffic(kernel):
@@ -96,30 +85,33 @@ def measured_traffic(kernel):
 
 
 def cpu_baseline(P, nthreads):
+    """CPU rate of the oracle on this host (a -march=native build when it
+    builds, else the portable one) and, from the portable build that every
+    check is pinned on, the reference outputs and the FP64 noise floor."""
     from oracle import oracle as O
     try:
-        O.use_native()
-        kind_build = "native"
+        O.native_lib()
+        native, kind_build = True, "native"
     except Exception:
-        kind_build = "portable"
+        native, kind_build = False, "portable"
     args = (P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
-    O.elemental(*args, nthreads=nthreads)                     # warm-up
+    O.elemental(*args, nthreads=nthreads, native=native)     # warm-up
     runs = []
-    out = None
     for _ in range(5):
         t0 = time.perf_counter()
-        out = O.elemental(*args, nthreads=nthreads)
+        O.elemental(*args, nthreads=nthreads, native=native)
         runs.append(time.perf_counter() - t0)
+    out = O.elemental(*args, nthreads=nthreads)
     t_all = float(np.median(runs))
     # single core on a 2048-element sample (the MEX element loop is single-threaded)
     ns = min(2048, P.ne)
     sample = (P.master, P.app, P.mesh.dgnodes[:, :, :ns], P.mesh.bf[:, :ns], P.udg[:, :, :ns],
               P.uh[:, :, :ns], P.sh[:, :, :ns])
-    O.elemental(*sample, nthreads=1)
+    O.elemental(*sample, nthreads=1, native=native)
     r1 = []
     for _ in range(5):
         t0 = time.perf_counter()
-        O.elemental(*sample, nthreads=1)
+        O.elemental(*sample, nthreads=1, native=native)
         r1.append(time.perf_counter() - t0)
     t_one = float(np.median(r1))
     # FP64 noise floor of this workload: the oracle's own response to random
@@ -347,10 +339,19 @@ def uneven_leg(torch, total, seed, steps):
 
 
 def tets_leg(torch, n, seed, steps):
-    """3D Euler p=2 on 6 n^3 jittered tetrahedra (u-only, FP64): elemental
-    throughput with the outputs resident in HBM."""
+    """3D Euler p=2 on 6 n^3 jittered tetrahedra (u-only): elemental throughput
+    in FP64 and FP32 + FP64 refinement, with the outputs resident in HBM.  The
+    step runs in element-matrix workspace chunks; a sample of elements -- both
+    sides of every chunk boundary and 64 drawn from the seed -- is checked
+    against the oracle (FP64) and its floor, the oracle's response to +-1 ulp
+    in node coordinates, state and trace.  On the far-field faces parallel to
+    the free stream u.n is zero to rounding, where the Roe |A|'s derivative
+    takes sign(u.n): coordinates rounded differently flip that sign and move
+    H by about 1e-10 there, which the coordinate perturbation puts in the
+    floor."""
     from hdg_amd.problems import euler3d_cube
-    from hdg_amd.runtime import Context, Dev
+    from hdg_amd.runtime import Context, Dev, PREC_FP32_IR, PREC_FP64
+    from oracle import oracle as O
     t0 = time.perf_counter()
     P = euler3d_cube(n, 2, seed=seed)
     log(f"[bench] tets: {P.ne} elements built in {time.perf_counter() - t0:.1f}s")
@@ -360,13 +361,87 @@ def tets_leg(torch, n, seed, steps):
     npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.ndf, P.ne
     out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
            Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
-    ms, wall = time_steps(torch, ctx.stream, lambda: ctx.elemental(P.master, P.app, *dev, out=out), steps, 1)
-    bad = int(np.count_nonzero(out[4].numpy()))
-    res = {"workload": f"euler3d_hdg_elemental_p2_6x{n}^3", "elements": ne, "ms_per_step": ms,
-           "value": ne / (ms * 1e-3), "unit": "elements/s", "status_nonzero": bad}
-    log(f"[bench] tets: {ms:.2f} ms/step, {ne / ms * 1e3:.4g} elements/s")
+    # chunk boundaries of the default 24 GiB workspace (as hdgx_ws_chunk)
+    per = 8 * (ncu * npv * npv * nc + ncu * npv * ncu * ndf + ncu * ndf * npv * nc + (ncu * ndf) ** 2
+               + ncu * npv + ncu * ndf)
+    cmax = max(1, min(ne, (24 << 30) // per))
+    nch = -(-ne // cmax)
+    chunk = -(-ne // nch)
+    rng = np.random.default_rng(seed)
+    idx = set(rng.choice(ne, size=min(64, ne), replace=False).tolist())
+    for b in range(chunk, ne, chunk):
+        idx.update({b - 2, b - 1, b, b + 1})
+    idx.update({0, ne - 1})
+    idx = np.array(sorted(i for i in idx if 0 <= i < ne))
+    sl = lambda a: np.asfortranarray(a[..., idx])
+    args = (P.master, P.app, sl(P.mesh.dgnodes), sl(P.mesh.bf), sl(P.udg), sl(P.uh), sl(P.sh))
+    ref = O.elemental(*args)
+    ulp = lambda a: np.asfortranarray(a * (1.0 + np.ldexp(1.0, -52) * rng.choice([-1.0, 1.0], size=a.shape)))
+    dg1 = args[2].copy(order="F")
+    dg1[:, :3, :] = ulp(dg1[:, :3, :])
+    pert = O.elemental(P.master, P.app, dg1, args[3], ulp(args[4]), ulp(args[5]), args[6])
+    names = ("AE", "FE", "DUDG", "DUDG_DUH")
+    floor = {nm: relerr(pert[i], ref[i]) for i, nm in enumerate(names)}
+    res = {"workload": f"euler3d_hdg_elemental_p2_6x{n}^3", "elements": ne, "unit": "elements/s",
+           "workspace_chunks": nch, "chunk_elements": chunk,
+           "oracle_sample": f"{idx.size} elements: both sides of each chunk boundary, the first, the last, 64 random",
+           "sample_noise_floor": floor}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_FP32_IR, "fp32_ir")):
+        ctx.set_precision(prec)
+        ms, wall = time_steps(torch, ctx.stream, lambda: ctx.elemental(P.master, P.app, *dev, out=out), steps, 1)
+        sel = torch.from_numpy(idx).cuda()
+        got = [o.t.view(ne, -1)[sel].cpu().numpy().T for o in out[:4]]      # (per element, sample)
+        errs = {nm: relerr(g.reshape(r.shape, order="F"), r) for nm, g, r in zip(names, got, ref[:4])}
+        bad = int(np.count_nonzero(out[4].numpy()))
+        entry = {"ms_per_step": ms, "value": ne / (ms * 1e-3), "status_nonzero": bad, "rel_err_sample_vs_oracle": errs}
+        if name == "fp64":
+            res.update(entry)
+        else:
+            res["fp32_ir"] = entry
+            res["fp32_ir"]["speedup_vs_fp64"] = res["ms_per_step"] / ms
+        log(f"[bench] tets {name}: {ms:.2f} ms/step, {ne / ms * 1e3:.4g} elements/s, sample errors {errs}")
+    ctx.set_precision(PREC_FP64)
+    ctx.set_timing(True)
+    ctx.elemental(P.master, P.app, *dev, out=out)
+    tq, ts, nl = ctx.kernel_times()
+    ctx.set_timing(False)
+    res["kernels_ms"] = {"quad_kernel": tq, "elimination": ts, "launch_pairs": nl}
     del out, dev
     torch.cuda.empty_cache()
+    return res
+
+
+def euler_c1_leg(torch, seed, steps, warmup):
+    """2D Euler p=2 on the jittered 2x32x32 square (2048 triangles, u-only,
+    schur_u), FP64 and FP32 + refinement, each against the oracle."""
+    from hdg_amd.problems import euler2d_square
+    from hdg_amd.runtime import Context, Dev, PREC_FP32_IR, PREC_FP64
+    from hdg_amd.workload import problem_work
+    from oracle import oracle as O
+    P = euler2d_square(32, 2, seed=seed)
+    W = problem_work(P)
+    ctx = Context(0)
+    dev = (Dev.from_numpy(P.mesh.dgnodes), Dev.from_numpy(P.mesh.bf, np.int32), Dev.from_numpy(P.udg),
+           Dev.from_numpy(P.uh), Dev.from_numpy(P.sh))
+    npv, ncu, nc, ndf, ne = P.master.npv, P.app.ncu, P.app.nc, P.master.ndf, P.ne
+    out = (Dev.empty((ncu, ndf, ncu, ndf, ne)), Dev.empty((ncu, ndf, ne)), Dev.empty((npv, nc, ne)),
+           Dev.empty((npv, nc, ncu, ndf, ne)), Dev.empty((ne,), "int32"))
+    ref = O.elemental(P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    names = ("AE", "FE", "DUDG", "DUDG_DUH")
+    res = {"workload": "euler2d_hdg_elemental_p2_2x32x32", "elements": ne, "unit": "elements/s",
+           "bytes_per_element": W.bytes, "flops_per_element": W.flops}
+    for prec, name in ((PREC_FP64, "fp64"), (PREC_FP32_IR, "fp32_ir")):
+        ctx.set_precision(prec)
+        ms, _ = time_steps(torch, ctx.stream, lambda: ctx.elemental(P.master, P.app, *dev, out=out), steps, warmup)
+        errs = {nm: relerr(o.numpy(), r) for nm, o, r in zip(names, out[:4], ref[:4])}
+        entry = {"ms_per_step": ms, "value": ne / (ms * 1e-3), "rel_err_vs_oracle": errs,
+                 "hbm_gbs_algorithmic": W.bytes * ne / (ms * 1e-3) / 1e9}
+        if name == "fp64":
+            res.update(entry)
+        else:
+            res["fp32_ir"] = entry
+        log(f"[bench] C1 {name}: {ms * 1e3:.1f} us/step, {ne / ms * 1e3:.4g} elements/s, errors {errs}")
+    ctx.set_precision(PREC_FP64)
     return res
 
 
@@ -431,7 +506,7 @@ def main():
     ctx.set_precision(PREC_FP64)
 
     tq, ts = kernel_times(torch, ctx, P, dev, out, reps=5)
-    log(f"[bench] kernels (fp64): quad {tq:.3f} ms, schur {ts:.3f} ms")
+    log(f"[bench] kernels (fp64, events inside hdgx_elemental): quad {tq:.3f} ms, elimination {ts:.3f} ms")
     resid = residual_leg(torch, ctx, P, dev, a.steps, a.warmup)
     getq = getq_leg(torch, ctx, P, dev, a.steps, a.warmup)
     host = host_leg(torch, ctx, P)
@@ -450,9 +525,9 @@ def main():
     value = ne / (ms64 * 1e-3)
     # dominant kernel roofline (FP64 element Schur + quadrature are compute bound at this size)
     if ts >= tq:
-        kname, kflops, kms = "schur_mfma_kernel<NS2D p3>", W.schur_flops * ne, ts
+        kname, kflops, kms = "schur64_kernel<NS2D p3, tile layout>", W.schur_flops * ne, ts
     else:
-        kname, kflops, kms = "quad_kernel<NS2D p3>", W.quad_flops * ne, tq
+        kname, kflops, kms = "quad_kernel<NS2D p3, tile layout>", W.quad_flops * ne, tq
     achieved = kflops / (kms * 1e-3) / 1e12
     line = {
         "metric": METRIC,
@@ -473,11 +548,13 @@ def main():
                     "max_rel_err_vs_oracle": max(errs["fp32_ir"].values()), "rel_err": errs["fp32_ir"]},
         "fp64_rel_err_vs_oracle": errs["fp64"],
         "fp64_noise_floor": noise_floor,
+        "within_noise_floor": {p: {n: errs[p][n] <= max(1e-12 if p == "fp64" else 1e-10, noise_floor[n])
+                                   for n in names} for p in errs},
         "status_nonzero": status_bad,
         "step_rates": {"flops_tflops_fp64": W.flops * ne / (ms64 * 1e-3) / 1e12,
                        "hbm_gbs_algorithmic_fp64": W.bytes * ne / (ms64 * 1e-3) / 1e9,
                        "bytes_per_element": W.bytes, "flops_per_element": W.flops},
-        "kernels_ms": {"quad_kernel": tq, "schur_kernel": ts},
+        "kernels_ms": {"quad_kernel": tq, "elimination_kernel": ts},
         "roofline": {"kernel": kname, "bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": measured_traffic(kname),
                      "flops_per_element": (W.schur_flops if ts >= tq else W.quad_flops)},
@@ -493,6 +570,7 @@ def main():
     }
     del out, dev
     torch.cuda.empty_cache()
+    line["c1_euler2d_p2"] = euler_c1_leg(torch, a.seed, max(a.steps, 20), a.warmup)
     if a.serve_requests > 0:
         line["serving"] = serving_leg(a.serve_requests, a.seed)
     if a.uneven > 0:

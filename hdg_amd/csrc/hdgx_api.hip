@@ -309,9 +309,23 @@ int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, con
       SchurArgs sa{n, W.M, W.C, W.E, W.BD, W.F, W.GK, W.H, W.Rq, W.Ru, W.Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
                    nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
                    status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0]};
+      if (ctx->timing) HIP_TRY(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
       int r = launch_quad<Cc>(ctx, mb, ap, qa, tile);
       if (r) return r;
-      return launch_schur<Cc>(ctx, sa, mb.tab.perm, tile);
+      if (ctx->timing) HIP_TRY(ctx, hipEventRecord(ctx->tev[1], ctx->stream));
+      r = launch_schur<Cc>(ctx, sa, mb.tab.perm, tile);
+      if (r) return r;
+      if (ctx->timing) {
+        HIP_TRY(ctx, hipEventRecord(ctx->tev[2], ctx->stream));
+        HIP_TRY(ctx, hipEventSynchronize(ctx->tev[2]));
+        float q = 0.f, el = 0.f;
+        HIP_TRY(ctx, hipEventElapsedTime(&q, ctx->tev[0], ctx->tev[1]));
+        HIP_TRY(ctx, hipEventElapsedTime(&el, ctx->tev[1], ctx->tev[2]));
+        ctx->tsum[0] += q;
+        ctx->tsum[1] += el;
+        ++ctx->tcalls;
+      }
+      return (int)HDGX_OK;
     });
     if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
     if (rc) return rc;
@@ -454,6 +468,8 @@ int hdgx_destroy(hdgx_ctx* ctx) {
   if (ctx->hws) (void)hipFree(ctx->hws);
   if (ctx->mws) (void)hipFree(ctx->mws);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (hipEvent_t ev : ctx->tev)
+    if (ev) (void)hipEventDestroy(ev);
   delete ctx;
   return HDGX_OK;
 }
@@ -483,6 +499,25 @@ int hdgx_set_workspace_limit(hdgx_ctx* ctx, int64_t bytes) {
     ctx->ws = nullptr;
     ctx->ws_bytes = 0;
   }
+  return HDGX_OK;
+}
+
+int hdgx_set_timing(hdgx_ctx* ctx, int on) {
+  if (!ctx) return HDGX_ERR_ARG;
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  for (int i = 0; i < 3 && on; ++i)
+    if (!ctx->tev[i]) HIP_TRY(ctx, hipEventCreate(&ctx->tev[i]));
+  ctx->timing = on != 0;
+  ctx->tsum[0] = ctx->tsum[1] = 0.0;
+  ctx->tcalls = 0;
+  return HDGX_OK;
+}
+
+int hdgx_kernel_times(hdgx_ctx* ctx, double* quad_ms, double* elim_ms, int64_t* launches) {
+  if (!ctx || !quad_ms || !elim_ms) return HDGX_ERR_ARG;
+  *quad_ms = ctx->tsum[0];
+  *elim_ms = ctx->tsum[1];
+  if (launches) *launches = ctx->tcalls;
   return HDGX_OK;
 }
 

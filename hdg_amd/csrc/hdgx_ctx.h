@@ -36,6 +36,12 @@ struct hdgx_ctx {
   size_t hws_bytes = 0;
   void* mws = nullptr;           // mesh-connectivity scratch (hash table, scan)
   size_t mws_bytes = 0;
+  // kernel timing of hdgx_elemental (hdgx_set_timing): events around the
+  // quadrature and elimination launches, summed milliseconds
+  bool timing = false;
+  hipEvent_t tev[3] = {nullptr, nullptr, nullptr};
+  double tsum[2] = {0.0, 0.0};
+  int64_t tcalls = 0;
 };
 
 namespace hdgx_detail {

@@ -44,6 +44,8 @@ def lib():
         L.hdgx_set_precision.argtypes = [vp, i32]
         L.hdgx_set_flags.argtypes = [vp, i32]
         L.hdgx_set_workspace_limit.argtypes = [vp, i64]
+        L.hdgx_set_timing.argtypes = [vp, i32]
+        L.hdgx_kernel_times.argtypes = [vp, vp, vp, vp]
         L.hdgx_last_error.argtypes = [vp]
         L.hdgx_last_error.restype = ctypes.c_char_p
         L.hdgx_elemental.argtypes = [vp, vp, vp, i64] + [vp] * 10
@@ -78,7 +80,7 @@ def lib():
 
 
 EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision", "hdgx_set_flags",
-           "hdgx_set_workspace_limit", "hdgx_last_error",
+           "hdgx_set_workspace_limit", "hdgx_set_timing", "hdgx_kernel_times", "hdgx_last_error",
            "hdgx_elemental", "hdgx_elemental_matrices", "hdgx_residual", "hdgx_getq", "hdgx_schur_u", "hdgx_schur_uq",
            "hdgx_elemental_batch", "hdgx_elemental_batch_mixed", "hdgx_elemental_noalloc",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
@@ -180,6 +182,16 @@ class Context:
         """hdgx_set_workspace_limit: most bytes of the element-matrix
         workspace (0 = the 24 GiB default); larger calls run in chunks."""
         self._check(lib().hdgx_set_workspace_limit(self.h, int(nbytes)))
+
+    def set_timing(self, on):
+        """hdgx_set_timing: events around the elemental kernels (synchronous calls)."""
+        self._check(lib().hdgx_set_timing(self.h, int(bool(on))))
+
+    def kernel_times(self):
+        """(quadrature ms, elimination ms, launch pairs) summed since set_timing."""
+        q, el, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        self._check(lib().hdgx_kernel_times(self.h, ctypes.byref(q), ctypes.byref(el), ctypes.byref(n)))
+        return q.value, el.value, n.value
 
     def _check(self, rc):
         if rc:

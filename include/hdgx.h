@@ -129,6 +129,14 @@ int hdgx_set_flags(hdgx_ctx* ctx, int flags);
  * first call that needs it, grows up to the limit, and is held until
  * hdgx_destroy (or freed here when it is above a new, lower limit). */
 int hdgx_set_workspace_limit(hdgx_ctx* ctx, int64_t bytes);
+/* Kernel timing of later hdgx_elemental / hdgx_elemental_noalloc calls on ctx
+ * (no counterpart in the reference; for measurement): with on != 0 each
+ * chunk's quadrature and elimination launches are bracketed by HIP events on
+ * the context's stream and the call synchronises to add their elapsed times;
+ * hdgx_kernel_times returns the sums (milliseconds) and the number of
+ * launch pairs since the last hdgx_set_timing. */
+int hdgx_set_timing(hdgx_ctx* ctx, int on);
+int hdgx_kernel_times(hdgx_ctx* ctx, double* quad_ms, double* elim_ms, int64_t* launches);
 const char* hdgx_last_error(hdgx_ctx* ctx);
 
 /* Elemental operator.  Replaces

@@ -12,42 +12,64 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
-LIB_NATIVE = os.path.join(HERE, "liboracle_native.so")
 
 _lib = None
 
 
-def build(force=False, native=False):
-    target = "native" if native else "all"
-    if force or not os.path.exists(LIB_NATIVE if native else LIB):
-        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []) + [target])
+def build(force=False):
+    if force or not os.path.exists(LIB):
+        subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []) + ["all"])
 
 
-def use_native():
-    """Switch to the -march=native build (for CPU timing on this host)."""
-    global _lib
-    build(native=True)
-    _lib = None
-    return lib(LIB_NATIVE)
+_native = None
 
 
-def lib(path=None):
+def native_lib():
+    """A -march=native build of the same source, for CPU timing only, built
+    for the host it runs on into a temporary folder keyed by that host's CPU
+    (a build made on another host is never loaded).  The process-wide
+    library that every check uses stays the portable build: the two differ
+    where an FP64 branch turns on rounding (|u.n| at u.n = 0 on a far-field
+    face parallel to the flow), and checks are pinned on the portable one."""
+    global _native
+    if _native is None:
+        import hashlib
+        import platform
+        import tempfile
+        cpu = platform.processor() + platform.machine()
+        try:
+            with open("/proc/cpuinfo") as fh:
+                cpu += "".join(l for l in fh if l.startswith(("model name", "flags")))
+        except OSError:
+            pass
+        key = hashlib.sha1(cpu.encode()).hexdigest()[:12]
+        out = os.path.join(tempfile.gettempdir(), f"hdgx_oracle_native_{key}", "liboracle_native.so")
+        os.makedirs(os.path.dirname(out), exist_ok=True)
+        subprocess.check_call(["make", "-s", "-C", HERE, "native", f"NATIVE_OUT={out}"])
+        _native = _bind(ctypes.CDLL(out))
+    return _native
+
+
+def _bind(L):
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
+    L.oracle_elemental.argtypes = [vp, vp, i64] + [vp] * 10 + [i32]
+    L.oracle_elemental_matrices.argtypes = [vp, vp, i64] + [vp] * 15 + [i32]
+    L.oracle_schur_u.argtypes = [i32] * 4 + [i64] + [vp] * 11 + [i32]
+    L.oracle_schur_uq.argtypes = [i32] * 5 + [i64] + [vp] * 15 + [i32]
+    L.oracle_schur_uq2.argtypes = [i32] * 5 + [i64] + [vp] * 15
+    L.oracle_flux.argtypes = [vp, i32, vp, vp, vp, vp]
+    L.oracle_fhat.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]
+    L.oracle_fbou.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp]
+    return L
+
+
+def lib():
+    """The portable build, which every check uses."""
     global _lib
     if _lib is None:
-        path = path or LIB
-        if not os.path.exists(path):
+        if not os.path.exists(LIB):
             build()
-        L = ctypes.CDLL(path)
-        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64
-        L.oracle_elemental.argtypes = [vp, vp, i64] + [vp] * 10 + [i32]
-        L.oracle_elemental_matrices.argtypes = [vp, vp, i64] + [vp] * 15 + [i32]
-        L.oracle_schur_u.argtypes = [i32] * 4 + [i64] + [vp] * 11 + [i32]
-        L.oracle_schur_uq.argtypes = [i32] * 5 + [i64] + [vp] * 15 + [i32]
-        L.oracle_schur_uq2.argtypes = [i32] * 5 + [i64] + [vp] * 15
-        L.oracle_flux.argtypes = [vp, i32, vp, vp, vp, vp]
-        L.oracle_fhat.argtypes = [vp, i32, vp, vp, vp, vp, vp, vp, vp]
-        L.oracle_fbou.argtypes = [vp, i32, vp, i32, vp, vp, vp, vp, vp, vp, vp]
-        _lib = L
+        _lib = _bind(ctypes.CDLL(LIB))
     return _lib
 
 
@@ -65,8 +87,9 @@ def _ref(app):
     return app if isinstance(app, CApp) else app.to_c()
 
 
-def elemental(master, app, dgnodes, bf, udg, uh, sh=None, nthreads=0):
-    """[AE, FE, DUDG, DUDG_DUH, status] = hdg_elemental(...) on the CPU."""
+def elemental(master, app, dgnodes, bf, udg, uh, sh=None, nthreads=0, native=False):
+    """[AE, FE, DUDG, DUDG_DUH, status] = hdg_elemental(...) on the CPU
+    (native=True: the -march=native build, for timing)."""
     from hdg_amd.app import MasterC
     mc = MasterC(master)
     ca = _ref(app)
@@ -81,7 +104,8 @@ def elemental(master, app, dgnodes, bf, udg, uh, sh=None, nthreads=0):
     dudg = np.zeros((npv, nc, ne), order="F")
     dudg_duh = np.zeros((npv, nc, ncu, ndf, ne), order="F")
     st = np.zeros(ne, dtype=np.int32)
-    rc = lib().oracle_elemental(ctypes.byref(mc.c), ctypes.byref(ca), ne, _p(dg), _p(b), _p(u), _p(h), _p(s),
+    L = native_lib() if native else lib()
+    rc = L.oracle_elemental(ctypes.byref(mc.c), ctypes.byref(ca), ne, _p(dg), _p(b), _p(u), _p(h), _p(s),
                                 _p(ae), _p(fe), _p(dudg), _p(dudg_duh), _p(st), nthreads)
     if rc:
         raise RuntimeError(f"oracle_elemental failed ({rc})")

@@ -125,3 +125,28 @@ def test_batch_chunked_equals_one_chunk(ctx, phys):
                 assert np.array_equal(a, b.numpy()), (P.name, name)
             else:
                 assert relerr(a, b.numpy()) < 1e-13, (P.name, name)
+
+
+def test_kernel_timing_counts_each_chunk(ctx):
+    """hdgx_set_timing brackets every chunk's quadrature and elimination with
+    events: one launch pair per chunk, positive times, outputs unchanged."""
+    P = euler3d_cube(3, 2, seed=41, alpha=0.3)
+    cap = 50
+    sizes = chunks_of(P.ne, cap)
+    print(f"[gpu] kernel timing euler3d_p2 ne={P.ne} chunks={sizes}", flush=True)
+    args = (P.master, P.app, P.mesh.dgnodes, P.mesh.bf, P.udg, P.uh, P.sh)
+    try:
+        ctx.set_workspace_limit(cap * element_bytes(P) + element_bytes(P) // 2)
+        plain = [o.numpy() for o in ctx.elemental(*args)]
+        ctx.set_timing(True)
+        timed = [o.numpy() for o in ctx.elemental(*args)]
+        tq, ts, n = ctx.kernel_times()
+        ctx.set_timing(False)
+        assert ctx.kernel_times() == (0.0, 0.0, 0)
+    finally:
+        ctx.set_timing(False)
+        ctx.set_workspace_limit(0)
+    assert n == len(sizes)
+    assert tq > 0 and ts > 0
+    for a, b in zip(timed, plain):
+        assert np.array_equal(a, b)
