@@ -47,8 +47,17 @@ hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, 
   if constexpr (S64Shape<C>::USE) {
     if (a.prec == 0) {
       if (tile && !a.aek) return hipErrorInvalidValue;
-      if (tile) hipLaunchKernelGGL((schur64_kernel<C, true>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
-      else hipLaunchKernelGGL((schur64_kernel<C, false>), dim3((unsigned)a.ne), dim3(64), 0, s, a);
+      if (S64Shape<C>::NW > 1 && a.status)
+        hipLaunchKernelGGL(s64_zero_status<C>, dim3((unsigned)((a.ne + 255) / 256)), dim3(256), 0, s, a.status, a.ne);
+      const dim3 grid((unsigned)s64_blocks<C>(a.ne));
+      if constexpr (WsLayout<C>::TILE_OK) {
+        if (tile) {
+          hipLaunchKernelGGL((schur64_kernel<C, true>), grid, dim3(64), 0, s, a);
+          return hipGetLastError();
+        }
+      }
+      if (tile) return hipErrorInvalidValue;
+      hipLaunchKernelGGL((schur64_kernel<C, false>), grid, dim3(64), 0, s, a);
       return hipGetLastError();
     }
   }
@@ -72,10 +81,19 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
   hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QUAD_THREADS), 0, s, t, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if constexpr (SchurLdsShape<C>::USE)
+  if constexpr (S64Shape<C>::USE) {
+    if constexpr (S64Shape<C>::NW > 1) {
+      hipLaunchKernelGGL(seg_zero_status<C>, dim3((unsigned)((t.total + 255) / 256)), dim3(256), 0, s, t);
+      e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)s64_blocks<C>(t.total)), dim3(64), 0, s, t, perm,
+                       m.aek[C::UQ ? 1 : 0]);
+  } else if constexpr (SchurLdsShape<C>::USE) {
     hipLaunchKernelGGL(schur_seg_lds_kernel<C>, dim3((unsigned)t.total), dim3(SCHUR_LDS_THREADS), 0, s, t);
-  else
+  } else {
     hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm, m.aek[C::UQ ? 1 : 0]);
+  }
   return hipGetLastError();
 }
 

@@ -20,7 +20,16 @@ struct S64Shape {
   static constexpr int CTD = DU / 16;            // its tile
   static constexpr int TRH = (NH + 15) / 16;     // RHS tiles = AE row tiles
   static constexpr int CT = RT + TRH;            // column tiles of the system
-  static constexpr bool USE = RT <= 3 && NU % 4 == 0 && NH % 4 == 0 && NU < R;
+  // four row tiles: the RHS tiles are split over NW one-wavefront blocks
+  // (each repeats the elimination of D' and carries TRW of them)
+  static constexpr int NW = RT == 4 ? (TRH >= 8 ? 4 : 2) : 1;
+  static constexpr int TRW = (TRH + NW - 1) / NW;
+  static constexpr int CTW = RT + TRW;          // column tiles one wavefront holds
+  // one to three row tiles with n_u a multiple of 4, or four row tiles of a
+  // u-only system (Euler p = 2 tetrahedra, n_u = 50; p = 4 triangles, 60) at
+  // one wavefront per SIMD, the accumulators spread over VGPRs and AGPRs
+  static constexpr bool USE = NH % 4 == 0 && NU < R &&
+                              ((RT <= 3 && NU % 4 == 0) || (RT == 4 && !C::UQ));
   static constexpr int KB = NPV * NCU * ND;      // columns of B (u,q)
   static constexpr int KBS = (KB + 3) / 4;
   static constexpr int KSV = (NPV + 3) / 4;      // k-steps of the q-row products
@@ -66,7 +75,7 @@ struct S64Shape {
 template <class C>
 struct WsLayout {
   using S = S64Shape<C>;
-  static constexpr bool TILE_OK = S::USE;
+  static constexpr bool TILE_OK = S::USE && S::RT <= 3;   // F's vectors take whole 4-row groups
   static constexpr size_t NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, ND = C::ND;
   static constexpr size_t bd = NU * NCOL, ru = NU, rh = NH;
   static constexpr size_t m = C::UQ ? NPV * NPV : 0, cm = C::UQ ? NPV * NPV * ND : 0;

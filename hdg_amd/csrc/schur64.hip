@@ -1,6 +1,7 @@
-// FP64 element elimination for the systems of one to three 16-row tiles
-// (kernel/schur_u.c:42-163, kernel/schur_uq.c:40-287): every 2D order p <= 3
-// and tetrahedra p = 1.  One wavefront per element, FP64 MFMA throughout.
+// FP64 element elimination for the systems of one to four 16-row tiles
+// (kernel/schur_u.c:42-163, kernel/schur_uq.c:40-287): every 2D order p <= 3,
+// tetrahedra p = 1 and, u-only at four row tiles, 2D Euler p = 4 and Euler
+// tetrahedra p = 2.  One wavefront per element, FP64 MFMA throughout.
 //
 // The element system lives in v_mfma_f64_16x16x4f64 accumulator tiles (lane
 // l = 16g + c, register i of a tile: row 4i + g, column c) as
@@ -83,7 +84,7 @@ template <class C> HDGX_DEV constexpr int s64_t(int j) { return j / C::NDF + C::
 template <class C>
 struct S64Lds {
   using S = S64Shape<C>;
-  static constexpr int R = S::R, RP = R + 1, CT = S::CT, NPV = C::NPV;
+  static constexpr int R = S::R, RP = R + 1, CT = S::CTW, NPV = C::NPV;
   static constexpr int RSTR = 16 * CT + 1;       // pivot row t at t*RSTR
   static constexpr int NCOLP = C::NCOL | 1;      // solution column cc at cc*NCOLP
   static constexpr int NHP = C::NH | 1;          // AE staging column cc at cc*NHP
@@ -105,15 +106,21 @@ struct S64Lds {
 
 // Element e of A (one 64-lane wavefront), LDS in L.  TILE: F, GK and H are
 // in the fragment order of WsLayout (hdgx_shapes.h) and A.aek is the master's
-// AE table; otherwise every array has the reference layout.
+// AE table; otherwise every array has the reference layout.  With four row
+// tiles the RHS tiles are split over S::NW wavefronts: `part` takes RHS tiles
+// part*TRW .. part*TRW + TRW - 1 (AE / DUDG_DUH columns), part 0 also du,
+// DUDG and FE; the status word is OR-ed into A.status[e], which the launch
+// zeroes first.
 template <class C, bool TILE = false>
-HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
+HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, S64Lds<C>& L) {
   using S = S64Shape<C>;
   using WL = WsLayout<C>;
   using LD = S64Lds<C>;
-  static_assert(S::USE, "schur64: one to three row tiles, NU a multiple of 4 below the padded size");
+  static_assert(S::USE, "schur64: one to four row tiles (four: u only), NU below the padded size");
   constexpr int NU = S::NU, NH = S::NH, NCOL = S::NCOL, NPV = S::NPV, NCU = S::NCU, ND = S::ND;
-  constexpr int NDF = S::NDF, NPF = S::NPF, NC = S::NC, RT = S::RT, R = S::R, TRH = S::TRH, CT = S::CT;
+  constexpr int NDF = S::NDF, NPF = S::NPF, NC = S::NC, RT = S::RT, R = S::R, TRH = S::TRH, CT = S::CTW;
+  constexpr int NW = S::NW, TRW = S::TRW;       // RHS tiles split over NW wavefronts, TRW each
+  static_assert(NW == 1 || !C::UQ, "the RHS split is for u-only systems");
   constexpr int DU = S::DU, CTD = S::CTD, KSMAX = S::KSMAX;
   constexpr bool UQ = C::UQ;
   constexpr int NQ = UQ ? NPV * NCU * ND : 0;
@@ -217,19 +224,20 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   }
   __syncthreads();
 
-  // the trace columns -F
+  // the trace columns -F of this wavefront's RHS tiles
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-    for (int tb = 0; tb < TRH; ++tb)
+    for (int tb = 0; tb < TRW; ++tb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = 16 * rt + 4 * i + g, j = 16 * tb + c;
+        const int tbg = part * TRW + tb;
+        const int row = 16 * rt + 4 * i + g, j = 16 * tbg + c;
         if constexpr (TILE) {
           const int ri = 4 * rt + i;
-          acc[rt][RT + tb][i] = ri < NU / 4 ? -Fe[(ri * TRH + tb) * 64 + lane] : 0.0;
+          acc[rt][RT + tb][i] = ri < NU / 4 ? -Fe[(ri * TRH + tbg) * 64 + lane] : 0.0;
         } else {
-          acc[rt][RT + tb][i] = (row < NU && j < NH) ? -Fe[row + NU * s64_t<C>(j)] : 0.0;
+          acc[rt][RT + tb][i] = (tbg < TRH && row < NU && j < NH) ? -Fe[row + NU * s64_t<C>(j)] : 0.0;
         }
       }
   // ---- D' += B M^-1 C, du's column -= B M^-1 Rq, RHS += B M^-1 E, over
@@ -293,6 +301,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
       for (int sub = 0; sub < 4; ++sub) {
         const int c0 = 4 * sub, k0 = 16 * ct0 + c0;
         if (k0 >= NU) break;
+        const int np = NU - k0 < 4 ? NU - k0 : 4;   // pivots of this panel (fewer in a last, partial one)
         {
           const int t = c - c0;
           if (t >= 0 && t < 4)
@@ -305,9 +314,10 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
         double pr[4], z[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int t = 0; t < 4; ++t) pr[t] = (lane < R) ? sPan[t * RP + lane] : 0.0;
-        int pvt[4];
+        int pvt[4] = {-1, -1, -1, -1};
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
+          if (t >= np) break;
           // |pr| as an ordered key whose low 6 bits carry 63 - lane: one
           // max-reduction finds the largest magnitude, ties to the lowest lane
           unsigned long long key = 0;
@@ -334,15 +344,23 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
           if (lane == 0) sStep[p] = k0 + t;
         }
 #pragma unroll
-        for (int t = 0; t < 4; ++t) if (lane == pvt[t]) z[t] -= 1.0;
+        for (int t = 0; t < 4; ++t) if (t < np && lane == pvt[t]) z[t] -= 1.0;
         if (lane < R) {
 #pragma unroll
           for (int t = 0; t < 4; ++t) sZ[t * RP + lane] = z[t];
         }
         // original (pre-panel) pivot rows, columns from tile ct0 on: lane group
-        // p & 3 owns row p in register (p & 15) >> 2 of row tile p >> 4
+        // p & 3 owns row p in register (p & 15) >> 2 of row tile p >> 4; the
+        // rows of a partial panel's missing pivots are zero (their Z column is)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
+          if (t >= np) {
+            if (g == 0)
+#pragma unroll
+              for (int ct = 0; ct < CT; ++ct)
+                if (ct >= ct0) sRows[t * RSTR + 16 * ct + c] = 0.0;
+            continue;
+          }
           const int p = pvt[t];
           const int sel = 4 * (p >> 4) + ((p & 15) >> 2);
           double tmp[CT];
@@ -375,7 +393,8 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   }
 
   // ---- du: column DU of D', solution row k at pivot row p_k; its q rows,
-  // DUDG and FE = Rh - GK x (schur_uq.c:146-186, 279-285)
+  // DUDG and FE = Rh - GK x (schur_uq.c:146-186, 279-285); part 0 only
+  if (part == 0) {
   if (c == DU % 16) {
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -436,6 +455,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
       A.fe[e * NH + row] = v;
     }
   }
+  }                                             // part 0
 
   // ---- A fragments of MiC for the q-row products (row n = l%16, k a = 4s + l/16)
   double mic[UQ ? ND : 1][S::KSV];
@@ -470,10 +490,12 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
   }
   __syncthreads();                            // sX (du) is reused below
 
-  // ---- per RHS tile
+  // ---- per RHS tile of this wavefront
 #pragma unroll
-  for (int tb = 0; tb < TRH; ++tb) {
-    const int j = 16 * tb + c;                 // this lane's RHS column
+  for (int tb = 0; tb < TRW; ++tb) {
+    const int tbg = part * TRW + tb;           // the tile's place among all TRH
+    if (tbg >= TRH) break;
+    const int j = 16 * tbg + c;                // this lane's RHS column
     // u rows in pivot order
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
@@ -492,7 +514,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
         double v = 0.0;
         if constexpr (TILE) {
           const int rh = 4 * rta + i;
-          if (rh < NH / 4) v = He[(tb * (NH / 4) + rh) * 64 + lane];
+          if (rh < NH / 4) v = He[(tbg * (NH / 4) + rh) * 64 + lane];
         } else if (row < NH && j < NH && (!sparse || row / (NCU * NPF) == (j % NDF) / NPF)) {
           v = He[row + NH * s64_t<C>(j)];
         }
@@ -530,7 +552,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
     if (LDO > 0) {
       double* Xe = A.dudg_duh + e * (int64_t)LDO * NH;
       for (int cc = 0; cc < 16; ++cc) {
-        const int jj = 16 * tb + cc;
+        const int jj = 16 * tbg + cc;
         if (jj >= NH) break;
         for (int r = lane; r < LDO; r += 64) {
           const double v = sX[r + NCOLP * cc];
@@ -590,7 +612,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
     double* AEe = A.ae + e * (int64_t)NH * NH;
     for (int idx = lane; idx < 16 * NH; idx += 64) {
       const int cc = idx / NH, row = idx - cc * NH;
-      const int jj = 16 * tb + cc;
+      const int jj = 16 * tbg + cc;
       if (jj < NH) {
         const double v = sX[row + NHP * cc];
         if (!isfinite(v)) st |= 4;
@@ -600,13 +622,41 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, S64Lds<C>& L) {
     __syncthreads();
   }
   st = wave_or(st);
-  if (lane == 0 && A.status) A.status[e] = st;
+  if (NW == 1) {
+    if (lane == 0 && A.status) A.status[e] = st;
+  } else if (lane == 0 && A.status && st) {
+    atomicOr(&A.status[e], st);
+  }
+}
+
+// blocks of schur64_kernel for ne elements: NW per element, the count rounded
+// to whole rounds of the 8 XCDs
+template <class C>
+__host__ __device__ constexpr int64_t s64_blocks(int64_t ne) {
+  constexpr int NW = S64Shape<C>::NW;
+  return NW == 1 ? ne : 8 * ((ne + 7) / 8) * NW;
 }
 
 template <class C, bool TILE>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S64Shape<C>::WPE, S64Shape<C>::WPE))) void schur64_kernel(SchurArgs A) {
   __shared__ S64Lds<C> lds;
-  schur64_body<C, TILE>(A, blockIdx.x, lds);
+  constexpr int NW = S64Shape<C>::NW;
+  if constexpr (NW == 1) {
+    schur64_body<C, TILE>(A, blockIdx.x, 0, lds);
+  } else {
+    // workgroups go round-robin over the 8 XCDs: block b runs on XCD b % 8,
+    // so an element's NW parts are blocks 8 k + x apart and share that XCD's
+    // L2 for the D' they all read
+    const int64_t b = blockIdx.x, x = b % 8, i = b / 8;
+    schur64_body<C, TILE>(A, (i / NW) * 8 + x, (int)(i % NW), lds);
+  }
+}
+
+// status words of ne elements cleared before a split elimination ORs into them
+template <class C>
+__global__ __launch_bounds__(256) void s64_zero_status(int32_t* st, int64_t ne) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i < ne) st[i] = 0;
 }
 
 }  // namespace hdgx

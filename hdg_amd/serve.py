@@ -221,10 +221,17 @@ class Server:
 
     budget_bytes: HBM the admitted requests may hold together; slots: maximum
     number of admitted requests; workers: concurrent trace solves (each with
-    its own hdgx context and stream)."""
+    its own hdgx context and stream).
+
+    Batching window: a request that becomes ready (admitted, or back from its
+    trace solve) waits for others until the oldest ready one has waited
+    batch_window seconds, the ready set holds batch_fill_elems elements, or
+    nothing that could join is in flight; then, once the previous elemental
+    call has finished, the ready set goes into one mixed-order call.  The
+    window bounds the latency batching adds per Newton step."""
 
     def __init__(self, device=0, budget_bytes=None, slots=64, workers=4, restart=20, gmres_tol=1e-6,
-                 max_batch_elems=2_000_000, admitters=2):
+                 max_batch_elems=2_000_000, admitters=2, batch_window=0.005, batch_fill_elems=200_000):
         torch = _torch()
         self.device = device
         if budget_bytes is None:
@@ -241,6 +248,8 @@ class Server:
         self.nadmit = admitters
         self.restart, self.gmres_tol = restart, gmres_tol
         self.max_batch_elems = max_batch_elems
+        self.batch_window = batch_window
+        self.batch_fill_elems = batch_fill_elems
         self.ctx = Context(device)
         self.main_stream = torch.cuda.current_stream(device)
 
@@ -383,6 +392,7 @@ class Server:
         used = 0
         active = []
         ready = []              # waiting for an elemental evaluation
+        ready_since = None      # when the oldest of them became ready
         finished = []
         nbatches = 0
         batched_elems = 0
@@ -410,10 +420,14 @@ class Server:
                     active.append(slot)
                     aq.put(slot)
                     inflight += 1
-                # one elemental call at a time (iteration-level batching): what
-                # becomes ready while the previous call still runs on the
-                # device joins the next one instead of queueing behind it alone
-                if ready and (last_call is None or last_call.query()):
+                # one elemental call at a time (iteration-level batching): the
+                # ready set closes when its window ends, it is large enough, or
+                # no admission or trace step could still join it; what becomes
+                # ready while the previous call still runs joins the next one
+                waited = time.perf_counter() - ready_since if ready else 0.0
+                full = sum(s.spec.ne for s in ready) >= self.batch_fill_elems
+                close = ready and (waited >= self.batch_window or full or inflight == 0)
+                if close and (last_call is None or last_call.query()):
                     for n, ne, norders in self._elemental_batches(ready):
                         nbatches += 1
                         batched_elems += ne
@@ -422,7 +436,7 @@ class Server:
                     for s in ready:
                         wq.put(s)
                         inflight += 1
-                    ready = []
+                    ready, ready_since = [], None
                 # collect admissions and finished trace steps; wait when nothing
                 # else can progress
                 timeout = None
@@ -430,8 +444,10 @@ class Server:
                     timeout = max(0.0, pending[0].t_arrival - clock()) if realtime else 0.0
                 if inflight == 0 and timeout is None:
                     timeout = 0.0
-                if ready:                   # the previous call is still running: poll
-                    timeout = 2e-4 if timeout is None else min(timeout, 2e-4)
+                if ready:                   # the window's end, or the previous call: poll
+                    left = max(self.batch_window - (time.perf_counter() - ready_since), 0.0) if not close else 0.0
+                    wait = max(min(left, 2e-4) if not close else 2e-4, 1e-5)
+                    timeout = wait if timeout is None else min(timeout, wait)
                 try:
                     item = doneq.get(timeout=timeout) if (timeout is None or timeout > 0) else doneq.get_nowait()
                 except queue.Empty:
@@ -441,10 +457,11 @@ class Server:
                     inflight -= 1
                     if exc is not None:
                         raise exc
-                    if kind == "built":
-                        admit_s += slot.admit_s
-                        ready.append(slot)
-                    elif slot.it < slot.spec.iters:
+                    if kind == "built" or slot.it < slot.spec.iters:
+                        if kind == "built":
+                            admit_s += slot.admit_s
+                        if not ready:
+                            ready_since = time.perf_counter()
                         ready.append(slot)
                     else:
                         slot.t_done = time.perf_counter() - t0
@@ -486,6 +503,7 @@ class Server:
             "gmres_iters_mean": float(np.mean([np.mean(s.gmres_iters) for s in finished])) if finished else None,
             "admission_ms_mean": admit_s / len(finished) * 1e3 if finished else None,
             "mixed_order_batches": mixed_batches,
+            "batch_window_ms": self.batch_window * 1e3,
         }
         return stats, finished
 
@@ -498,9 +516,10 @@ def main():
     ap.add_argument("--workers", type=int, default=4)
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--physics", default=PHYS_NS, choices=[PHYS_NS, PHYS_EULER])
+    ap.add_argument("--batch-window", type=float, default=0.005, help="seconds a ready request waits for others")
     a = ap.parse_args()
     specs = make_workload(a.requests, a.seed, a.rate, physics=a.physics)
-    srv = Server(workers=a.workers, slots=a.slots)
+    srv = Server(workers=a.workers, slots=a.slots, batch_window=a.batch_window)
     stats, _ = srv.run(specs, log=lambda m: print(m, file=sys.stderr, flush=True))
     print(json.dumps(stats))
 

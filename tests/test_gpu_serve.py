@@ -62,12 +62,13 @@ def test_served_result_is_the_same_alone_and_in_the_mix():
     specs = _small_workload()
     fac = PayloadFactory()
     print("[gpu] serve mix", flush=True)
-    srv = Server(workers=3, slots=16)
+    # a batching window far longer than the admissions: the 8 requests, of
+    # orders 1..4, are admitted together and their first elemental call is
+    # shared across orders
+    srv = Server(workers=3, slots=16, batch_window=0.5)
     stats, done = srv.run(specs, factory=fac, realtime=False, keep_state=True)
     assert stats["requests"] == len(specs)
     mix = {s.spec.rid: (s.state.UDG.numpy(), s.state.UH.numpy()) for s in done}
-    # the 8 requests, of orders 1..4, are admitted together; requests whose
-    # admission finishes together share an elemental call across orders
     print(f"[gpu]   {stats['elemental_batches']} batches, {stats['mean_requests_per_batch']:.2f} requests each, "
           f"{stats['mixed_order_batches']} over several orders", flush=True)
     assert stats["elemental_batches"] < sum(s.iters for s in specs)
