@@ -308,7 +308,7 @@ int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, con
                   W.BD, W.F, W.GK, W.H, W.Ru, W.Rh, W.M, W.C, W.E, W.Rq, nullptr};
       SchurArgs sa{n, W.M, W.C, W.E, W.BD, W.F, W.GK, W.H, W.Rq, W.Ru, W.Rh, nco ? dudg + (size_t)e0 * ldo : nullptr,
                    nco ? dudg_duh + (size_t)e0 * ldo * nh : nullptr, ae + (size_t)e0 * nh * nh, fe + (size_t)e0 * nh,
-                   status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0]};
+                   status ? status + e0 : nullptr, ctx->prec, nco, mb.tab.aek[uq ? 1 : 0], 1};
       if (ctx->timing) HIP_TRY(ctx, hipEventRecord(ctx->tev[0], ctx->stream));
       int r = launch_quad<Cc>(ctx, mb, ap, qa, tile);
       if (r) return r;

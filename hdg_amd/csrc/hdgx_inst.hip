@@ -7,6 +7,7 @@
 #include "getq_kernel.hip"
 #include "schur_mfma.hip"
 #include "schur64.hip"
+#include "schur_inv.hip"
 #include "schur_lds.hip"
 #include "seg_kernels.hip"
 
@@ -44,13 +45,30 @@ hipError_t getq_launch(hipStream_t s, const GetqArgs& a, const DevMaster& t) {
 
 template <class C>
 hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, bool tile) {
+  if constexpr (InvShape<C>::USE) {
+    // the library's own workspace: D^-1 replaces D, then the MFMA products
+    // (face blocks with the compact layout)
+    if (a.prec == 0 && a.bd_scratch) {
+      constexpr bool FB = WsLayout<C>::FB;
+      if (tile && !FB) return hipErrorInvalidValue;
+      if (tile) hipLaunchKernelGGL((inv64_kernel<C, FB>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
+      else hipLaunchKernelGGL((inv64_kernel<C, false>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      if (tile)
+        hipLaunchKernelGGL(apply64_fb_kernel<C>, dim3((unsigned)apply64_blocks<C, true>(a.ne)), dim3(64), 0, s, a, perm);
+      else
+        hipLaunchKernelGGL(apply64_kernel<C>, dim3((unsigned)apply64_blocks<C>(a.ne)), dim3(64), 0, s, a);
+      return hipGetLastError();
+    }
+  }
   if constexpr (S64Shape<C>::USE) {
     if (a.prec == 0) {
       if (tile && !a.aek) return hipErrorInvalidValue;
       if (S64Shape<C>::NW > 1 && a.status)
         hipLaunchKernelGGL(s64_zero_status<C>, dim3((unsigned)((a.ne + 255) / 256)), dim3(256), 0, s, a.status, a.ne);
       const dim3 grid((unsigned)s64_blocks<C>(a.ne));
-      if constexpr (WsLayout<C>::TILE_OK) {
+      if constexpr (WsLayout<C>::TILE_OK && !WsLayout<C>::FB) {
         if (tile) {
           hipLaunchKernelGGL((schur64_kernel<C, true>), grid, dim3(64), 0, s, a);
           return hipGetLastError();
@@ -81,7 +99,13 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
   hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QUAD_THREADS), 0, s, t, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
-  if constexpr (S64Shape<C>::USE) {
+  if constexpr (InvShape<C>::USE) {
+    constexpr bool FB = WsLayout<C>::FB;      // the batched workspace is compact where a layout exists
+    hipLaunchKernelGGL(inv_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, m.perm);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(apply_seg_kernel<C>, dim3((unsigned)apply64_blocks<C, FB>(t.total)), dim3(64), 0, s, t, m.perm);
+  } else if constexpr (S64Shape<C>::USE) {
     if constexpr (S64Shape<C>::NW > 1) {
       hipLaunchKernelGGL(seg_zero_status<C>, dim3((unsigned)((t.total + 255) / 256)), dim3(256), 0, s, t);
       e = hipGetLastError();

@@ -100,6 +100,7 @@ struct SchurArgs {
   int32_t prec;                    // 0 fp64, 1 fp32 mfma + fp64 refinement
   int32_t nco;                     // DUDG / DUDG_DUH rows kept per node: nc, ncu or 0
   const unsigned char* aek;        // AE k-step table (DevMaster::aek) when GK is face sparse, else null
+  int32_t bd_scratch;              // 1: BD is the library's workspace and its D block may be overwritten
 };
 
 // One request's element range inside a batched launch: its arrays and the

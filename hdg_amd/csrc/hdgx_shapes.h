@@ -57,6 +57,24 @@ struct S64Shape {
   static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
 };
 
+// Face-block form of the u-only four-row-tile systems whose faces pad to whole
+// 16-row tiles (Euler tetrahedra p = 2: 30 trace rows per face, two tiles):
+// F, GK and H are nonzero only where a trace row or column and a volume node
+// share a face (faceint.m:316-359; H block diagonal per face), so each is kept
+// as NFE blocks of NB x NB, NB = NCU * NPF.  Face-local indices: trace r =
+// k + NCU b (component k, face node b: trace column f NB + r), volume
+// q = a + NPF m (face node a, i.e. volume node perm(a, f), component m: u row
+// perm(a, f) + NPV m).
+template <class C>
+struct FbShape {
+  using S = S64Shape<C>;
+  static constexpr int NB = C::NCU * C::NPF;
+  static constexpr int NBT = (NB + 15) / 16;     // 16-row tiles per face
+  static constexpr int KSF = (NB + 3) / 4;       // face-local k-steps
+  static constexpr bool USE = S::USE && S::RT == 4 && !C::UQ && NBT * C::NFE <= S::TRH;
+  static constexpr size_t BLK = (size_t)C::NFE * NB * NB;   // words of F, GK or H
+};
+
 // Per-element words of the element-matrix workspace in hdgx_elemental and the
 // batched launches.  The reference layout keeps every array as
 // kernel/schur_primal.m receives it; the tile layout (FP64 configurations of
@@ -71,18 +89,26 @@ struct S64Shape {
 //   H  (TRH, NH/4) vectors: lane 16g + c of vector (tb, rh) = H(row 4 rh + g,
 //      trace column j = 16 tb + c);
 // trace column j = jf + NDF k is H's / F's column k + NCU jf, zero past NH.
+// For FbShape configurations the compact layout is the face-block one:
+//   F  block f: (q, r) at q + NB r;  GK block f: (r, q) at r + NB q;
+//   H  block f: (r, r') at r + NB r'.
 // The other arrays (BD, Ru, Rh, M, C, E, Rq) keep the reference layout.
 template <class C>
 struct WsLayout {
   using S = S64Shape<C>;
-  static constexpr bool TILE_OK = S::USE && S::RT <= 3;   // F's vectors take whole 4-row groups
+  static constexpr bool FB = FbShape<C>::USE;
+  static constexpr bool TILE_OK = S::USE && (S::RT <= 3 || FB);   // fragment tiles or face blocks
   static constexpr size_t NU = C::NU, NH = C::NH, NCOL = C::NCOL, NPV = C::NPV, ND = C::ND;
   static constexpr size_t bd = NU * NCOL, ru = NU, rh = NH;
   static constexpr size_t m = C::UQ ? NPV * NPV : 0, cm = C::UQ ? NPV * NPV * ND : 0;
   static constexpr size_t em = C::UQ ? NPV * C::NDF * ND : 0, rq = C::UQ ? NPV * C::NCU * ND : 0;
-  static constexpr size_t f(bool tile) { return tile ? (NU / 4) * S::TRH * 64 : NU * NH; }
-  static constexpr size_t gk(bool tile) { return tile ? (size_t)S::KSP * S::TRH * 64 : NH * NCOL; }
-  static constexpr size_t h(bool tile) { return tile ? S::TRH * (NH / 4) * 64 : NH * NH; }
+  static constexpr size_t f(bool tile) {
+    return tile ? (FB ? FbShape<C>::BLK : (NU / 4) * S::TRH * 64) : NU * NH;
+  }
+  static constexpr size_t gk(bool tile) {
+    return tile ? (FB ? FbShape<C>::BLK : (size_t)S::KSP * S::TRH * 64) : NH * NCOL;
+  }
+  static constexpr size_t h(bool tile) { return tile ? (FB ? FbShape<C>::BLK : S::TRH * (NH / 4) * 64) : NH * NH; }
   static constexpr size_t words(bool tile) { return bd + f(tile) + gk(tile) + h(tile) + ru + rh + m + cm + em + rq; }
   static constexpr size_t max_words() {
     return TILE_OK && words(true) > words(false) ? words(true) : words(false);

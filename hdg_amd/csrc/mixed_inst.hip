@@ -6,6 +6,7 @@
 #include "quad_kernel.hip"
 #include "schur_mfma.hip"
 #include "schur64.hip"
+#include "schur_inv.hip"
 #include "schur_lds.hip"
 #include "seg_kernels.hip"
 #include "mixed_kernels.hip"

@@ -85,6 +85,7 @@ HDGX_DEV void schur_mixed_elem(const MixedTable& T, const ElemSeg& sg, int64_t e
   A.prec = 0;
   A.nco = C::NC;
   A.aek = nullptr;
+  A.bd_scratch = 0;
   schur_mfma_body<C>(A, T.masters[P].perm, 0, lds);
 }
 

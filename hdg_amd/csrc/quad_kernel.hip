@@ -629,7 +629,16 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE) {
+    if constexpr (TILE && FbShape<C>::USE) {
+      // face blocks: F_f(q, r) = F(u row of q on face f, trace column f NB + r)
+      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+      double* F = A.F + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int f = idx / (NB * NB), rem = idx - f * NB * NB, q = rem % NB, r = rem / NB;
+        const int va = T.perm[q % NPF + NPF * f];
+        F[idx] = f_entry(va, q / NPF, r % NCU, f * NPF + r / NCU);
+      }
+    } else if constexpr (TILE) {
       // the elimination's fragment order (hdgx_shapes.h, WsLayout)
       using S = S64Shape<C>;
       constexpr int NV = (NU / 4) * S::TRH * 64;
@@ -661,7 +670,16 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE) {
+    if constexpr (TILE && FbShape<C>::USE) {
+      // face blocks: GK_f(r, q) = GK(trace row f NB + r, u column of q on face f)
+      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+      double* GK = A.GK + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int f = idx / (NB * NB), rem = idx - f * NB * NB, r = rem % NB, q = rem / NB;
+        const int va = T.perm[q % NPF + NPF * f];
+        GK[idx] = gk_entry(r % NCU, f * NPF + r / NCU, va, q / NPF);
+      }
+    } else if constexpr (TILE) {
       // A fragments of the AE product over the face-sparse columns (the
       // master's AE k-step table)
       using S = S64Shape<C>;
@@ -694,7 +712,15 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE) {
+    if constexpr (TILE && FbShape<C>::USE) {
+      // face blocks: H_f(r, r') = H(f NB + r, f NB + r')
+      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+      double* H = A.H + e * (int64_t)NV;
+      for (int idx = tid; idx < NV; idx += NT) {
+        const int f = idx / (NB * NB), rem = idx - f * NB * NB, r = rem % NB, r2 = rem / NB;
+        H[idx] = h_entry(r % NCU, f * NPF + r / NCU, r2 % NCU, f * NPF + r2 / NCU);
+      }
+    } else if constexpr (TILE) {
       using S = S64Shape<C>;
       constexpr int NHV = NH / 4, NV = S::TRH * NHV * 64;
       double* H = A.H + e * (int64_t)NV;

@@ -107,6 +107,7 @@ HDGX_DEV SchurArgs seg_schur_args(const SegTable& S, int64_t w) {
   A.prec = 0;
   A.nco = C::NC;
   A.aek = nullptr;
+  A.bd_scratch = 1;                // the launch's workspace
   return A;
 }
 
@@ -130,10 +131,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>()
   if constexpr (S64Shape<C>::USE) {
     A.aek = aek;
     __shared__ S64Lds<C> lds;
-    schur64_body<C, WsLayout<C>::TILE_OK>(A, 0, part, lds);
+    schur64_body<C, WsLayout<C>::TILE_OK && !WsLayout<C>::FB>(A, 0, part, lds);
   } else {
     __shared__ SchurLds<C> lds;
     schur_mfma_body<C>(A, perm, 0, lds);
+  }
+}
+
+// the four-row-tile u-only configurations: D^-1 over the workspace's D, then
+// the MFMA products (schur_inv.hip), as hdgx_elemental does
+template <class C>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void inv_seg_kernel(SegTable S,
+                                                                                               const int32_t* perm) {
+  if constexpr (InvShape<C>::USE) {
+    const int64_t w = blockIdx.x;
+    if (w >= S.total) return;
+    const SchurArgs A = seg_schur_args<C>(S, w);
+    __shared__ InvLds<C> lds;
+    inv64_body<C, WsLayout<C>::FB>(A, 0, perm, lds);
+  }
+}
+
+template <class C>
+__global__ __launch_bounds__(64) void apply_seg_kernel(SegTable S, const int32_t* perm) {
+  if constexpr (InvShape<C>::USE) {
+    constexpr bool FB = WsLayout<C>::FB;
+    constexpr int NW = FB ? C::NFE : InvShape<C>::NWB;
+    const int64_t b = blockIdx.x, x = b % 8, i = b / 8;
+    const int64_t w = (i / NW) * 8 + x;
+    if (w >= S.total) return;
+    const SchurArgs A = seg_schur_args<C>(S, w);
+    if constexpr (FB) {
+      __shared__ ApplyFbLds<C> lds;
+      apply64_fb_body<C>(A, 0, (int)(i % NW), perm, lds);
+    } else {
+      apply64_body<C>(A, 0, (int)(i % NW));
+    }
   }
 }
 
