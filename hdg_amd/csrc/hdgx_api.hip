@@ -168,7 +168,6 @@ int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) 
   if (a->physics == HDGX_PHYS_EULER) {
     if (a->nc != a->ncu || a->localsolve != 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "Euler: u-only (localsolve 0, nc = ncu)");
     if (a->nparam < 3) return fail(ctx, HDGX_ERR_ARG, "Euler: param needs {gam, eps, ..., tau}");
-    if (nd == 2 && a->param[1] > 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "Euler 2D far field with eps > 0");
   } else if (a->physics == HDGX_PHYS_NS) {
     if (nd != 2 || a->nc != 12 || a->localsolve != 1) return fail(ctx, HDGX_ERR_UNSUPPORTED, "NS: 2D with u,q (nc 12, localsolve 1)");
     if (a->nparam < 6) return fail(ctx, HDGX_ERR_ARG, "NS: param needs {gam, eps, Re, Pr, Minf, tau}");
@@ -181,7 +180,7 @@ int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) 
   std::memset(d, 0, sizeof(*d));
   d->gam = a->param[0];
   d->eps = a->param[1];
-  if (a->physics == HDGX_PHYS_NS) { d->Re = a->param[2]; d->Pr = a->param[3]; }
+  if (a->physics == HDGX_PHYS_NS) { d->Re = a->param[2]; d->Pr = a->param[3]; d->Minf = a->param[4]; }
   d->tau = a->param[a->nparam - 1];
   d->fc_u = a->fc_u;
   d->fc_q = a->fc_q;
@@ -191,7 +190,11 @@ int make_app(hdgx_ctx* ctx, const hdgx_master* m, const hdgx_app* a, DevApp* d) 
   d->nbc = a->nbc;
   for (int i = 0; i < a->nbc; ++i) {
     const int ib = a->bcm[i];
-    const bool ok = ib == 1 || (ib == 2 && a->physics == HDGX_PHYS_EULER && nd == 2);
+    // far field everywhere; 2D Euler: wall 2, prescribed pressure 3 and 4,
+    // inflow 5; NS: adiabatic wall 2, isothermal wall 3, prescribed pressure 4,
+    // Dirichlet 6
+    const bool euler2 = a->physics == HDGX_PHYS_EULER && nd == 2, ns = a->physics == HDGX_PHYS_NS;
+    const bool ok = ib == 1 || (euler2 && ib >= 2 && ib <= 5) || (ns && ib >= 2 && ib <= 4) || (ns && ib == 6);
     if (!ok) return fail(ctx, HDGX_ERR_UNSUPPORTED, "boundary model " + std::to_string(ib) + " not implemented");
     d->bcm[i] = ib;
     for (int c = 0; c < a->ncu; ++c) d->bcs[i * 5 + c] = a->bcs[i * HDGX_MAX_PARAM + c];

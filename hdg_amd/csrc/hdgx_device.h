@@ -65,6 +65,18 @@ HDGX_DEV double dsqrt(double a) { return sqrt(a); }
 HDGX_DEV double dabs(double a) { return fabs(a); }
 template <int N> HDGX_DEV double val(const Dual<N>& a) { return a.v; }
 HDGX_DEV double val(double a) { return a; }
+// euler/getan.m:97-103 with eps > 0: l <- 0.5 (l^2/(eps c) + eps c), and its
+// derivative lines written with that new l where the old one stands:
+// l' <- l l'/(eps c) + 0.5 (1 - l^2/(eps c)^2) eps c'
+HDGX_DEV double llf_smooth(double l, double c, double eps) { const double ec = eps * c; return 0.5 * (l * l / ec + ec); }
+template <int N> HDGX_DEV Dual<N> llf_smooth(const Dual<N>& l, const Dual<N>& c, double eps) {
+  const double ec = eps * c.v;
+  Dual<N> r;
+  r.v = 0.5 * (l.v * l.v / ec + ec);
+#pragma unroll
+  for (int i = 0; i < N; ++i) r.d[i] = r.v * l.d[i] / ec + 0.5 * (1.0 - r.v * r.v / (ec * ec)) * (eps * c.d[i]);
+  return r;
+}
 
 // type of an expression in operands of types A and B (double or one Dual type):
 // double only when both are

@@ -52,6 +52,7 @@ struct DevMaster {
 // Physics parameters passed by value.
 struct DevApp {
   double gam, eps, Re, Pr, tau;
+  double Minf;                     // NS free-stream Mach number (isothermal wall)
   double fc_u, fc_q;
   int32_t tdep, av_col, ncd, nbc;
   int32_t bcm[8];

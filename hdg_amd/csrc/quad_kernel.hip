@@ -155,26 +155,64 @@ HDGX_DEV void face_fhat(const DevApp& ap, double av, const double* n, const TU* 
   }
 }
 
-// Boundary flux: far field (ib 1) fh = (an + An)(u - uh) - (an - An)(uinf - uh)
-// = an (u - uinf) + An (u + uinf - 2 uh) with the LLF pair an = |un| + c,
-// An = un - c in 2D Euler (euler/fbou.m:24-29, getan.m) and the Roe pair
-// an = A_n, An = |A_n| for NS and 3D Euler (ns/fbou.m:37-46,
-// fbou_euler3d.c:20-60); 2D Euler inviscid wall (ib 2: euler/fbou.m:30-48).
-// Host validation rejects other boundary types.
+// Boundary flux.  The far-field form fh = (an + An)(u - uh) - (an - An)(ub - uh)
+// = an (u - ub) + An (u + ub - 2 uh), with the LLF pair an = |un| + c,
+// An = un - c in 2D Euler (euler/fbou.m:24-29, getan.m; eps > 0 smooths an as
+// getan.m:97-103 writes it) and the Roe pair an = A_n, An = |A_n| for NS and
+// 3D Euler (ns/fbou.m:37-46, fbou_euler3d.c:20-60), takes ub = uinf for the far
+// field (ib 1) and the prescribed-pressure state of u, ub = (r, ru, rv,
+// pb/(gam-1) + |ru|^2/(2r)) with pb = uinf(1) (Euler ib 3) or (uinf(4) - 0.5)
+// (gam - 1) (Euler and NS ib 4: euler/fbou.m:50-77, ns/fbou.m:76-90).  The
+// others: 2D Euler inviscid wall (ib 2, euler/fbou.m:30-48) and inflow (ib 5,
+// :78-91); NS adiabatic wall (ib 2, ns/fbou.m:47-71; its energy row is fhat's,
+// passed in fa), isothermal wall (ib 3, :72-95) and Dirichlet (ib 6,
+// :105-109).  Host validation rejects every other boundary type.
 template <class C, class TU, class TH>
 HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const double* n, const TU* u, const TH* uh,
-                        prom_t<TU, TH> fh[C::NCU]) {
+                        const prom_t<TU, TH>* fa, prom_t<TU, TH> fh[C::NCU]) {
   constexpr int ND = C::ND, NCU = C::NCU;
   typedef prom_t<TU, TH> T;
-  if (ib == 2) {
-    if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
+  if constexpr (ND == 2) {
+    if (C::PHYS == PHYS_EULER && ib == 2) {
       TU un = u[1] * n[0] + u[2] * n[1];
       fh[0] = u[0] - uh[0];
       fh[1] = (u[1] - n[0] * un) - uh[1];
       fh[2] = (u[2] - n[1] * un) - uh[2];
       fh[3] = u[3] - uh[3];
+      return;
     }
-    return;
+    if (C::PHYS == PHYS_EULER && ib == 5) {
+      fh[0] = u[0] - uh[0];
+      fh[1] = uinf[1] - uh[1];
+      fh[2] = uinf[2] - uh[2];
+      fh[3] = u[3] - uh[3];
+      return;
+    }
+    if (C::PHYS == PHYS_NS && (ib == 2 || ib == 3)) {
+      fh[0] = u[0] - uh[0];
+      fh[1] = 0.0 - uh[1];
+      fh[2] = 0.0 - uh[2];
+      if (ib == 2) fh[3] = fa[3];
+      else fh[3] = ap.gam * (ap.gam - 1.0) * ap.Minf * ap.Minf * uh[3] / uh[0] - uinf[3];
+      return;
+    }
+    if (C::PHYS == PHYS_NS && ib == 6) {
+#pragma unroll
+      for (int k = 0; k < NCU; ++k) fh[k] = uh[k] - uinf[k];
+      return;
+    }
+  }
+  // the far-field form with ub = uinf or the prescribed-pressure state of u
+  TU ub[NCU];
+#pragma unroll
+  for (int k = 0; k < NCU; ++k) ub[k] = uinf[k];
+  if constexpr (ND == 2) {
+    if (ib == 3 || ib == 4) {
+      const double pb = ib == 3 ? uinf[0] : (uinf[3] - 0.5) * (ap.gam - 1.0);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) ub[k] = u[k];
+      ub[3] = pb / (ap.gam - 1.0) + 0.5 * (u[1] * u[1] + u[2] * u[2]) / u[0];
+    }
   }
   if constexpr (C::PHYS == PHYS_EULER && ND == 2) {
     const double gam1 = ap.gam - 1.0;
@@ -185,16 +223,17 @@ HDGX_DEV void face_fbou(const DevApp& ap, int ib, const double* uinf, const doub
     TH c = dsqrt(ap.gam * p * r1);
     TH un = uv * n[0] + vv * n[1];
     TH l2 = dabs(un) + c, l3 = un - c;
+    if (ap.eps > 0) l2 = llf_smooth(l2, c, ap.eps);
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) fh[k] = l2 * (u[k] - uinf[k]) + l3 * ((u[k] - uh[k]) + (uinf[k] - uh[k]));
+    for (int k = 0; k < NCU; ++k) fh[k] = l2 * (u[k] - ub[k]) + l3 * ((u[k] - uh[k]) + (ub[k] - uh[k]));
   } else {
     TU x0[NCU];
     T x[NCU], y[NCU];
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) x0[k] = u[k] - uinf[k];
+    for (int k = 0; k < NCU; ++k) x0[k] = u[k] - ub[k];
     roe_apply<ND>(ap.gam, ap.eps, 0, uh, n, x0, y);
 #pragma unroll
-    for (int k = 0; k < NCU; ++k) { fh[k] = y[k]; x[k] = (u[k] - uh[k]) + (uinf[k] - uh[k]); }
+    for (int k = 0; k < NCU; ++k) { fh[k] = y[k]; x[k] = (u[k] - uh[k]) + (ub[k] - uh[k]); }
     roe_apply<ND>(ap.gam, ap.eps, 1, uh, n, x, y);
 #pragma unroll
     for (int k = 0; k < NCU; ++k) fh[k] = fh[k] + y[k];
@@ -216,7 +255,10 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
     double uinf[NCU];
 #pragma unroll
     for (int c = 0; c < NCU; ++c) uinf[c] = ap.bcs[id * 5 + c];
-    face_fbou<C>(ap, ap.bcm[id], uinf, nl, u, uh, fh);
+    prom_t<TU, TH> fa_[NCU];
+#pragma unroll
+    for (int c = 0; c < NCU; ++c) fa_[c] = fh[c];
+    face_fbou<C>(ap, ap.bcm[id], uinf, nl, u, uh, fa_, fh);
   }
 #pragma unroll
   for (int c = 0; c < NCU; ++c) fb[c] = fh[c];

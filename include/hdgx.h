@@ -80,7 +80,11 @@ typedef struct hdgx_app {
   int32_t nparam;
   double param[HDGX_MAX_PARAM]; /* Euler {gam, eps, tau}; NS {gam, eps, Re, Pr, Minf, tau} */
   int32_t nbc;
-  int32_t bcm[HDGX_MAX_BC];                 /* boundary model per boundary id */
+  /* boundary model per boundary id, the reference's fbou cases: 1 far field
+   * (all); 2D Euler (euler/fbou.m): 2 inviscid wall, 3 and 4 prescribed
+   * pressure, 5 inflow; NS (ns/fbou.m): 2 adiabatic wall, 3 isothermal wall,
+   * 4 prescribed pressure, 6 Dirichlet.  Others: HDGX_ERR_UNSUPPORTED. */
+  int32_t bcm[HDGX_MAX_BC];
   double bcs[HDGX_MAX_BC * HDGX_MAX_PARAM]; /* bcs[i*HDGX_MAX_PARAM + c] = state of id i+1 */
 } hdgx_app;
 

@@ -156,8 +156,13 @@ void roe_an(int nd, double gam, double eps, int absflag, const T* uh, const doub
 
 // Euler 2D local Lax-Friedrichs diagonals (euler/getan.m:25-141): absolute 2
 // gives |un|+c, absolute 3 gives un-c.
-template <class T>
-void llf_lam(double gam, const T* uh, const double* n, T& lam2, T& lam3) {
+// With eps > 0 (getan.m:97-103) |un|+c becomes 0.5 (l^2/(eps c) + eps c) and
+// its derivative lines use that new value where l is written:
+//   l' <- lnew l'/(eps c) + 0.5 (1 - lnew^2/(eps c)^2) eps c'
+// (the chain rule would take the old l there); restated as written.
+template <int N>
+void llf_lam(double gam, double eps, const Dn<N>* uh, const double* n, Dn<N>& lam2, Dn<N>& lam3) {
+  typedef Dn<N> T;
   const double gam1 = gam - 1.0;
   T r1 = 1.0 / uh[0];
   T uv = uh[1] * r1, vv = uh[2] * r1;
@@ -167,6 +172,14 @@ void llf_lam(double gam, const T* uh, const double* n, T& lam2, T& lam3) {
   T un = uv * n[0] + vv * n[1];
   lam2 = dabs(un) + c;
   lam3 = un - c;
+  if (eps > 0) {
+    const double ec = eps * c.v;
+    T l = lam2;
+    l.v = 0.5 * (lam2.v * lam2.v / ec + ec);
+    for (int i = 0; i < N; ++i)
+      l.d[i] = l.v * lam2.d[i] / ec + 0.5 * (1.0 - l.v * l.v / (ec * ec)) * (eps * c.d[i]);
+    lam2 = l;
+  }
 }
 
 struct Phys {
@@ -241,21 +254,39 @@ void fhat_point(const Phys& P, const double* nl, const double* pg, const double*
   }
 }
 
-// Boundary flux (euler/fbou.m:21-92, ns/fbou.m:37-46, fbou_euler3d.c:20-60).
+// Boundary flux (euler/fbou.m:21-92, ns/fbou.m:37-101, fbou_euler3d.c:20-60).
 // Returns false for a boundary model this oracle does not restate.
 bool fbou_point(const Phys& P, int ib, const double* uinf, const double* nl, const double* pg,
                 const double* udg, const double* uh, double* fh, double* fh_udg, double* fh_uh) {
-  (void)pg;
   const int ncu = P.ncu, nd = P.nd, nc = P.nc;
   for (int i = 0; i < ncu * nc; ++i) fh_udg[i] = 0.0;
-  if (ib == 1) {  // far field
+  const bool euler2 = P.physics == HDGX_PHYS_EULER && nd == 2, ns = P.physics == HDGX_PHYS_NS;
+  // the far-field form (an + An)(u - uh) - (an - An)(ub - uh) with ub = uinf
+  // (ib 1) or the prescribed-pressure state of u: ub = (r, ru, rv, pb/(gam-1)
+  // + |ru|^2/(2 r)), pb = uinf(1) (euler ib 3) or (uinf(4) - 0.5)(gam - 1)
+  // (euler ib 4, euler/fbou.m:50-77; ns ib 4, ns/fbou.m:76-90)
+  const bool pres = nd == 2 && ((euler2 && (ib == 3 || ib == 4)) || (ns && ib == 4));
+  if (ib == 1 || pres) {
+    double ub[MAXU], Ub[MAXU][MAXU];           // ub and dub/du
+    for (int i = 0; i < ncu; ++i) {
+      ub[i] = pres ? udg[i] : uinf[i];
+      for (int k = 0; k < ncu; ++k) Ub[i][k] = (pres && i == k) ? 1.0 : 0.0;
+    }
+    if (pres) {
+      const double gam = P.param[0], r = udg[0], ru = udg[1], rv = udg[2];
+      const double pb = ib == 3 ? uinf[0] : (uinf[3] - 0.5) * (gam - 1.0);
+      ub[3] = pb / (gam - 1.0) + 0.5 * (ru * ru + rv * rv) / r;
+      Ub[3][0] = -0.5 * (ru * ru + rv * rv) / (r * r);
+      Ub[3][1] = ru / r;
+      Ub[3][2] = rv / r;
+      Ub[3][3] = 0.0;
+    }
     Dn<MAXU> w[MAXU];
     for (int c = 0; c < ncu; ++c) w[c] = Dn<MAXU>::var(uh[c], c);
     Dn<MAXU> an[MAXU][MAXU], An[MAXU][MAXU];
-    if (P.physics == HDGX_PHYS_EULER && nd == 2) {
-      if (P.param[1] > 0) return false;  // euler/getan.m eps>0 LLF branch not restated
+    if (euler2) {
       Dn<MAXU> l2, l3;
-      llf_lam(P.param[0], w, nl, l2, l3);
+      llf_lam(P.param[0], P.param[1], w, nl, l2, l3);
       for (int i = 0; i < ncu; ++i)
         for (int k = 0; k < ncu; ++k) { an[i][k] = Dn<MAXU>((i == k) ? 0.0 : 0.0); An[i][k] = an[i][k]; }
       for (int i = 0; i < ncu; ++i) { an[i][i] = l2; An[i][i] = l3; }
@@ -266,10 +297,12 @@ bool fbou_point(const Phys& P, int ib, const double* uinf, const double* nl, con
     for (int i = 0; i < ncu; ++i) {
       Dn<MAXU> s(0.0);
       for (int k = 0; k < ncu; ++k)
-        s = s + (an[i][k] + An[i][k]) * (udg[k] - w[k]) - (an[i][k] - An[i][k]) * (uinf[k] - w[k]);
+        s = s + (an[i][k] + An[i][k]) * (udg[k] - w[k]) - (an[i][k] - An[i][k]) * (ub[k] - w[k]);
       fh[i] = s.v;
       for (int k = 0; k < ncu; ++k) {
-        fh_udg[i + ncu * k] = an[i][k].v + An[i][k].v;
+        double v = an[i][k].v + An[i][k].v;
+        for (int j = 0; j < ncu; ++j) v -= (an[i][j].v - An[i][j].v) * Ub[j][k];
+        fh_udg[i + ncu * k] = v;
         fh_uh[i + ncu * k] = s.d[k];
       }
     }
@@ -287,6 +320,49 @@ bool fbou_point(const Phys& P, int ib, const double* uinf, const double* nl, con
     fh_udg[2 + 4 * 2] = 1.0 - nl[1] * nl[1];
     fh_udg[3 + 4 * 3] = 1.0;
     for (int c = 0; c < 4; ++c) fh_uh[c + 4 * c] = -1.0;
+    return true;
+  }
+  for (int i = 0; i < ncu * ncu; ++i) fh_uh[i] = 0.0;
+  if (ib == 5 && euler2) {  // euler/fbou.m:78-91: ub = (r, uinf(2:3), rE) of u
+    fh[0] = udg[0] - uh[0];
+    fh[1] = uinf[1] - uh[1];
+    fh[2] = uinf[2] - uh[2];
+    fh[3] = udg[3] - uh[3];
+    fh_udg[0 + 4 * 0] = 1.0;
+    fh_udg[3 + 4 * 3] = 1.0;
+    for (int c = 0; c < 4; ++c) fh_uh[c + 4 * c] = -1.0;
+    return true;
+  }
+  if (ib == 2 && ns) {  // adiabatic wall, ns/fbou.m:47-71: r - rh, -ruh, -rvh, fhat's energy row
+    double fa[MAXU], fa_udg[MAXU * MAXNC], fa_uh[MAXU * MAXU];
+    fhat_point(P, nl, pg, udg, uh, fa, fa_udg, fa_uh);
+    fh[0] = udg[0] - uh[0];
+    fh[1] = -uh[1];
+    fh[2] = -uh[2];
+    fh[3] = fa[3];
+    fh_udg[0] = 1.0;
+    for (int k = 0; k < nc; ++k) fh_udg[3 + ncu * k] = fa_udg[3 + ncu * k];
+    for (int c = 0; c < 3; ++c) fh_uh[c + ncu * c] = -1.0;
+    for (int k = 0; k < ncu; ++k) fh_uh[3 + ncu * k] = fa_uh[3 + ncu * k];
+    return true;
+  }
+  if (ib == 3 && ns) {  // isothermal wall, ns/fbou.m:72-95: energy row gam (gam-1) M^2 rEh/rh - uinf(end)
+    const double gam = P.param[0], M2 = P.param[4] * P.param[4], k = gam * (gam - 1.0) * M2;
+    fh[0] = udg[0] - uh[0];
+    fh[1] = -uh[1];
+    fh[2] = -uh[2];
+    fh[3] = k * uh[3] / uh[0] - uinf[ncu - 1];
+    fh_udg[0] = 1.0;
+    for (int c = 0; c < 3; ++c) fh_uh[c + ncu * c] = -1.0;
+    fh_uh[3 + ncu * 0] = -k * uh[3] / (uh[0] * uh[0]);
+    fh_uh[3 + ncu * 3] = k / uh[0];
+    return true;
+  }
+  if (ib == 6 && ns) {  // Dirichlet, ns/fbou.m:105-109: uh - uinf
+    for (int c = 0; c < ncu; ++c) {
+      fh[c] = uh[c] - uinf[c];
+      fh_uh[c + ncu * c] = 1.0;
+    }
     return true;
   }
   return false;
