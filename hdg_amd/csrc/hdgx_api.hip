@@ -789,6 +789,54 @@ int hdgx_elemental(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app
                             static_cast<double*>(ctx->ws), chunk);
 }
 
+int hdgx_local_solve(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                     const double* dgnodes, const int32_t* bf, const double* udg, const double* uh, const double* sh,
+                     const int32_t* elcon, const double* DUH, double* dudgt, int32_t* status) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if (ne < 0) return fail(ctx, HDGX_ERR_ARG, "ne < 0");
+  if (!master || !app) return fail(ctx, HDGX_ERR_ARG, "null master/app");
+  if (ctx->flags & HDGX_HOST_PTRS) return fail(ctx, HDGX_ERR_ARG, "local_solve: device arrays only");
+  if (ne > 0 && (!dgnodes || !bf || !udg || !uh || !elcon || !DUH || !dudgt)) return fail(ctx, HDGX_ERR_ARG, "null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  MasterBuf* mb = nullptr;
+  int rc = get_master(ctx, master, &mb);
+  if (rc) return rc;
+  // the element solves keep every row (schur_primal's dudg, dudg_duh)
+  hdgx_app full = *app;
+  full.getdqdg = 1;
+  DevApp ap;
+  if ((rc = make_app(ctx, master, &full, &ap))) return rc;
+  if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
+  if (ne == 0) return HDGX_OK;
+  const size_t per = ws_words_of(app->physics, master->nd, porder_of(master->nd, master->npv));
+  if (per == 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "no kernel for this (physics, nd, p)");
+  const size_t npv = master->npv, nc = app->nc, ncu = app->ncu, nfe = master->nfe;
+  const size_t ndf = (size_t)master->npf * nfe, nh = ncu * ndf;
+  // per element: the element matrices, then AE, FE, DUDG, DUDG_DUH and a
+  // status word, all dropped once DUDGT is formed
+  const size_t outw = nh * nh + nh + npv * nc + npv * nc * nh + 1;
+  const int64_t chunk = ws_chunk(ne, (per + outw) * 8, ctx->ws_cap);
+  if ((rc = ensure_ws(ctx, (per + outw) * 8 * (size_t)chunk))) return rc;
+  double* const w = static_cast<double*>(ctx->ws);
+  double* const sAE = w + per * (size_t)chunk;
+  double* const sFE = sAE + nh * nh * (size_t)chunk;
+  double* const sDU = sFE + nh * (size_t)chunk;
+  double* const sDX = sDU + npv * nc * (size_t)chunk;
+  int32_t* const sST = reinterpret_cast<int32_t*>(sDX + npv * nc * nh * (size_t)chunk);
+  for (int64_t e0 = 0; e0 < ne; e0 += chunk) {
+    const int64_t n = std::min(chunk, ne - e0);
+    rc = elemental_launches(ctx, *mb, ap, master, &full, n, dgnodes + (size_t)e0 * npv * app->ncd,
+                            bf + (size_t)e0 * nfe, udg + (size_t)e0 * npv * nc, uh + (size_t)e0 * nh,
+                            sh ? sh + (size_t)e0 * npv * nc : nullptr, sAE, sFE, sDU, sDX, status ? status + e0 : sST,
+                            w, n);
+    if (rc) return rc;
+    rc = hdgx_local_update(ctx, (int)npv, (int)nc, (int)ncu, master->npf, master->nfe, n, elcon + (size_t)e0 * ndf,
+                           sDU, sDX, DUH, dudgt + (size_t)e0 * npv * nc);
+    if (rc) return rc;
+  }
+  return HDGX_OK;
+}
+
 int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
                            const double* dgnodes, const int32_t* bf, const double* udg, const double* uh,
                            const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,

@@ -70,6 +70,7 @@ def lib():
         L.hdgx_trace_scatter.argtypes = [vp, vp, vp, vp]
         L.hdgx_color_faces.argtypes = [i32, i64, vp, vp, vp]
         L.hdgx_local_update.argtypes = [vp, i32, i32, i32, i32, i32, i64, vp, vp, vp, vp, vp]
+        L.hdgx_local_solve.argtypes = [vp, vp, vp, i64] + [vp] * 9
         L.hdgx_newton_update.argtypes = [vp, i32, i32, i32, i64, i64, ctypes.c_double, vp, vp, vp, vp, vp, vp]
         L.hdgx_norm2.argtypes = [vp, i64, vp, vp]
         L.hdgx_mesh_faces.argtypes = [vp, i32, i64, i32, vp, vp, vp, vp]
@@ -85,7 +86,7 @@ EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision
            "hdgx_elemental_batch", "hdgx_elemental_batch_mixed", "hdgx_elemental_noalloc",
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
-           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_newton_update", "hdgx_norm2",
+           "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_local_solve", "hdgx_newton_update", "hdgx_norm2",
            "hdgx_mesh_faces", "hdgx_mesh_elcon", "hdgx_mesh_f2f"]
 
 
@@ -233,6 +234,20 @@ class Context:
         self._check(lib().hdgx_elemental(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
                                          h.ptr, _p(s), ae.ptr, fe.ptr, _p(dudg), _p(dudg_duh), st.ptr))
         return ae, fe, dudg, dudg_duh, st
+
+    def local_solve(self, master, app, dgnodes, bf, udg, uh, sh, elcon, DUH, out=None, status=None):
+        """hdgx_local_solve: DUDGT (npv, nc, ne) = dudg + dudg_duh DUH(:, elcon)
+        recomputed element by element (kernel/hdg_local.m); uh the gathered
+        trace (ncu, npf*nfe, ne), elcon (npf*nfe, ne) 0-based, DUH (ncu, nsiz)."""
+        ca = app.to_c() if isinstance(app, App) else app
+        mc = self.master_c(master)
+        dg, b, u, h, s = _dev(dgnodes), _dev(bf, "int32"), _dev(udg), _dev(uh), _dev(sh)
+        ec, dh = _dev(elcon, "int32"), _dev(DUH)
+        ne = dg.shape[2]
+        out = out or Dev.empty((master.npv, ca.nc, ne))
+        self._check(lib().hdgx_local_solve(self.h, ctypes.byref(mc.c), ctypes.byref(ca), ne, dg.ptr, b.ptr, u.ptr,
+                                           h.ptr, _p(s), ec.ptr, dh.ptr, out.ptr, _p(status)))
+        return out
 
     def elemental_noalloc(self, master, app, dgnodes, bf, udg, uh, sh, out):
         """hdgx_elemental_noalloc: elemental() with every array a device array

@@ -34,10 +34,11 @@ class HdgState:
         ndf = master.npf * master.nfe
         self.npv, self.nc, self.ncu, self.ne, self.ndf = npv, nc, ncu, ne, ndf
         # rows of DUDG kept by the elemental step (hdg_solve.m:56-60); with
-        # getdqdg = 0 q is recovered from (u, uh) by getq after every step
-        if app.getdqdg not in (0, 1):
-            raise HdgxError("getdqdg must be 0 or 1 (hdg_local's element solves are not built)")
+        # getdqdg = 0 q is recovered from (u, uh) by getq after every step;
+        # any other value keeps no DUDG / DUDG_DUH and the local update solves
+        # the element problems again (hdg_local.m, hdg_solve.m:122-124)
         self.getdqdg = app.getdqdg
+        self.local = app.getdqdg not in (0, 1)
         self.nco = ncu if app.getdqdg == 0 else nc
         nco = self.nco
         self.ts = TraceSystem(ctx, mesh, ncu, with_bilu0=with_bilu0)
@@ -51,8 +52,8 @@ class HdgState:
         self.uhe = Dev.empty((ncu, ndf, ne))
         self.AE = Dev.empty((ncu, ndf, ncu, ndf, ne))
         self.FE = Dev.empty((ncu, ndf, ne))
-        self.DUDG = Dev.empty((npv, nco, ne))
-        self.DUDG_DUH = Dev.empty((npv, nco, ncu, ndf, ne))
+        self.DUDG = None if self.local else Dev.empty((npv, nco, ne))
+        self.DUDG_DUH = None if self.local else Dev.empty((npv, nco, ncu, ndf, ne))
         self.est = Dev.empty((ne,), "int32")
         ts = self.ts
         self.K = Dev.empty((ts.ncf, ts.ncf, ts.nbf - 1, ts.nf))
@@ -90,7 +91,12 @@ class HdgState:
         return it, rr, fl
 
     def local_update(self):
-        self.ts.local_update(self.npv, self.nco, self.DUDG, self.DUDG_DUH, self.DUH, self.DUDGT)
+        if self.local:
+            # uhe still holds the gathered trace of this iteration's state
+            self.ctx.local_solve(self.master, self.app, self.dgnodes, self.bf, self.UDG, self.uhe, self.SH,
+                                 self.ts.elcon, self.DUH, out=self.DUDGT)
+        else:
+            self.ts.local_update(self.npv, self.nco, self.DUDG, self.DUDG_DUH, self.DUH, self.DUDGT)
 
     def step(self, alpha, base=True):
         """UDG = UDG0 + alpha DUDGT, UH = UH0 + alpha DUH (hdg_solve.m:145-146);

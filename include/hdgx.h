@@ -417,6 +417,20 @@ int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t ne, int64_t nf, const int32_t* 
  *   Rh (ncu, npf*nfe, ne), RH (ncu, npf*nf) in the layout of UH. */
 int hdgx_trace_scatter(hdgx_ctx* ctx, const hdgx_trace* T, const double* Rh, double* RH);
 
+/* Element-by-element local solves: DUDGT = dudg + dudg_duh * DUH(:, elcon)
+ * recomputed per element from the state, without DUDG_DUH for the whole mesh
+ * (replaces kernel/hdg_local.m:1-85, hdg_solve.m:122-124, the path of
+ * app.getdqdg outside {0, 1}).  Per workspace-sized chunk of elements it runs
+ * the elemental kernels (volint, faceint and the elimination: schur_primal's
+ * dudg, dudg_duh) into scratch and contracts them with the chunk's trace
+ * increments, as hdgx_local_update does.  Inputs as hdgx_elemental (uh the
+ * gathered trace (ncu, npf*nfe, ne)), elcon (npf*nfe, ne) 0-based, DUH
+ * (ncu, nsiz); DUDGT (npv, nc, ne) keeps every row; status (ne) optional.
+ * Device arrays only. */
+int hdgx_local_solve(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_app* app, int64_t ne,
+                     const double* dgnodes, const int32_t* bf, const double* udg, const double* uh, const double* sh,
+                     const int32_t* elcon, const double* DUH, double* dudgt, int32_t* status);
+
 /* Local update DUDGT = DUDG + DUDG_DUH * DUH(:, elcon) (kernel/hdg_solve.m:119-121,
  * util/mapContractK.m:47-99).  DUDG (npv,nco,ne), DUDG_DUH (npv,nco,ncu,ndf,ne),
  * DUH (ncu, nsiz), DUDGT (npv,nco,ne). */
