@@ -8,7 +8,6 @@
 #include "schur_mfma.hip"
 #include "schur64.hip"
 #include "schur_inv.hip"
-#include "schur_lds.hip"
 #include "seg_kernels.hip"
 
 namespace hdgx {
@@ -80,12 +79,6 @@ hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, 
     }
   }
   if (tile) return hipErrorInvalidValue;
-  if constexpr (SchurLdsShape<C>::USE) {
-    if (a.prec == 0) {
-      hipLaunchKernelGGL(schur_lds_kernel<C>, dim3((unsigned)a.ne), dim3(SCHUR_LDS_THREADS), 0, s, a);
-      return hipGetLastError();
-    }
-  }
   if (a.prec == 1)
     hipLaunchKernelGGL((schur_mfma_ir_kernel<C>), dim3((unsigned)a.ne), dim3(64), 0, s, a, perm);
   else
@@ -105,16 +98,6 @@ hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, cons
     e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(apply_seg_kernel<C>, dim3((unsigned)apply64_blocks<C, FB>(t.total)), dim3(64), 0, s, t, m.perm);
-  } else if constexpr (S64Shape<C>::USE) {
-    if constexpr (S64Shape<C>::NW > 1) {
-      hipLaunchKernelGGL(seg_zero_status<C>, dim3((unsigned)((t.total + 255) / 256)), dim3(256), 0, s, t);
-      e = hipGetLastError();
-      if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)s64_blocks<C>(t.total)), dim3(64), 0, s, t, perm,
-                       m.aek[C::UQ ? 1 : 0]);
-  } else if constexpr (SchurLdsShape<C>::USE) {
-    hipLaunchKernelGGL(schur_seg_lds_kernel<C>, dim3((unsigned)t.total), dim3(SCHUR_LDS_THREADS), 0, s, t);
   } else {
     hipLaunchKernelGGL(schur_seg_kernel<C>, dim3((unsigned)t.total), dim3(64), 0, s, t, perm, m.aek[C::UQ ? 1 : 0]);
   }

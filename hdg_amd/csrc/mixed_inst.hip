@@ -7,7 +7,6 @@
 #include "schur_mfma.hip"
 #include "schur64.hip"
 #include "schur_inv.hip"
-#include "schur_lds.hip"
 #include "seg_kernels.hip"
 #include "mixed_kernels.hip"
 

@@ -117,21 +117,15 @@ constexpr int seg_wpe() { return S64Shape<C>::USE ? S64Shape<C>::WPE : MfmaShape
 
 template <class C>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>(), seg_wpe<C>()))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm, const unsigned char* __restrict__ aek) {
-  constexpr int NW = S64Shape<C>::USE ? S64Shape<C>::NW : 1;
-  int64_t w = blockIdx.x;
-  int part = 0;
-  if constexpr (NW > 1) {
-    // as schur64_kernel: an element's NW parts on one XCD
-    const int64_t x = w % 8, i = w / 8;
-    w = (i / NW) * 8 + x;
-    part = (int)(i % NW);
-  }
+  // (the four-row-tile u-only configurations take inv_seg_kernel and
+  // apply_seg_kernel instead, so schur64 runs here one wavefront an element)
+  const int64_t w = blockIdx.x;
   if (w >= S.total) return;
   SchurArgs A = seg_schur_args<C>(S, w);
-  if constexpr (S64Shape<C>::USE) {
+  if constexpr (S64Shape<C>::USE && S64Shape<C>::NW == 1) {
     A.aek = aek;
     __shared__ S64Lds<C> lds;
-    schur64_body<C, WsLayout<C>::TILE_OK && !WsLayout<C>::FB>(A, 0, part, lds);
+    schur64_body<C, WsLayout<C>::TILE_OK>(A, 0, 0, lds);
   } else {
     __shared__ SchurLds<C> lds;
     schur_mfma_body<C>(A, perm, 0, lds);
@@ -170,24 +164,6 @@ __global__ __launch_bounds__(64) void apply_seg_kernel(SegTable S, const int32_t
   }
 }
 
-// the status words of a split elimination's elements, cleared first
-template <class C>
-__global__ __launch_bounds__(256) void seg_zero_status(SegTable S) {
-  const int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (w >= S.total) return;
-  const SchurArgs A = seg_schur_args<C>(S, w);
-  if (A.status) A.status[0] = 0;
-}
 
-// the configurations hdgx_elemental eliminates in LDS (SchurLdsShape<C>::USE)
-// take the same body here, so that a batch reproduces each request's own
-// call bit for bit
-template <class C>
-__global__ __launch_bounds__(SCHUR_LDS_THREADS) void schur_seg_lds_kernel(SegTable S) {
-  const int64_t w = blockIdx.x;
-  if (w >= S.total) return;
-  const SchurArgs A = seg_schur_args<C>(S, w);
-  schur_lds_body<C>(A, 0);
-}
 
 }  // namespace hdgx
