@@ -53,7 +53,8 @@ struct S64Shape {
   }
   static constexpr int KSP = ksp();
   // GK's A fragments stay in registers across the RHS tiles when they fit
-  static constexpr bool GKREG = KSP * TRH <= 16;
+  // (with three row tiles in the registers D' frees after the elimination)
+  static constexpr bool GKREG = KSP * TRH <= (RT == 3 ? 36 : 16);
   static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
 };
 

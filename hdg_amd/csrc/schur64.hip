@@ -246,9 +246,8 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
   if constexpr (UQ) {
     constexpr int BB = 4;
     constexpr S64Masks<C> SM = s64_masks<C>();
-#pragma unroll
-    for (int s0 = 0; s0 < S::KBS; s0 += BB) {
-      double a[BB][RT];
+    // B's A fragments a batch of k-steps ahead of the products that use them
+    auto load_b = [&](int s0, double (&a)[BB][RT]) {
 #pragma unroll
       for (int sb = 0; sb < BB; ++sb)
 #pragma unroll
@@ -256,6 +255,12 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
           const int row = 16 * rt + c, ka = 4 * (s0 + sb) + g;
           a[sb][rt] = (s0 + sb < S::KBS && row < NU && ka < S::KB) ? BDe[row + NU * (NU + ka)] : 0.0;
         }
+    };
+    double a[BB][RT], an[BB][RT];
+    load_b(0, a);
+#pragma unroll
+    for (int s0 = 0; s0 < S::KBS; s0 += BB) {
+      if (s0 + BB < S::KBS) load_b(s0 + BB, an);
 #pragma unroll
       for (int sb = 0; sb < BB; ++sb) {
         const int s = s0 + sb;
@@ -284,6 +289,10 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
           for (int rt = 0; rt < RT; ++rt) acc[rt][RT + tb] = mfma64(a[sb][rt], b, acc[rt][RT + tb]);
         }
       }
+#pragma unroll
+      for (int sb = 0; sb < BB; ++sb)
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) a[sb][rt] = an[sb][rt];
     }
   }
 
@@ -392,6 +401,26 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
       }
   }
 
+  // ---- GK's A fragments for FE and every RHS tile (row 16 rta + l%16,
+  // column of k-step s), in registers when they fit, loaded before FE needs them
+  constexpr int KSR = S::GKREG ? S::KSP : 1;
+  double gkf[KSR][TRH];
+  int kend[TRH];
+#pragma unroll
+  for (int rta = 0; rta < TRH; ++rta) kend[rta] = (__builtin_amdgcn_readfirstlane(L.sTab[rta]) + 3) / 4;
+  if constexpr (S::GKREG) {
+    if (sparse) {
+#pragma unroll
+      for (int s = 0; s < KSR; ++s)
+#pragma unroll
+        for (int rta = 0; rta < TRH; ++rta) {
+          const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
+          const int row = 16 * rta + c;
+          if constexpr (TILE) gkf[s][rta] = s < kend[rta] ? GKe[(s * TRH + rta) * 64 + lane] : 0.0;
+          else gkf[s][rta] = (s < kend[rta] && k != AE_NONE && row < NH) ? GKe[row + NH * k] : 0.0;
+        }
+    }
+  }
   // ---- du: column DU of D', solution row k at pivot row p_k; its q rows,
   // DUDG and FE = Rh - GK x (schur_uq.c:146-186, 279-285); part 0 only
   if (part == 0) {
@@ -425,11 +454,18 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
     // four lane groups are added
 #pragma unroll
     for (int rta = 0; rta < TRH; ++rta) {
-      const int kl = (__builtin_amdgcn_readfirstlane(L.sTab[rta]) + 3) / 4;
+      // (the register fragments, or loads unrolled to the tile's bound so
+      // they are in flight together)
       double v = 0.0;
-      for (int s = 0; s < kl; ++s) {
-        const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
-        if (k != AE_NONE) v += GKe[(s * TRH + rta) * 64 + lane] * sX[k];
+#pragma unroll
+      for (int s = 0; s < S::KSP; ++s) {
+        if (s < kend[rta]) {
+          const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
+          double gv;
+          if constexpr (S::GKREG) gv = gkf[s < KSR ? s : 0][rta];
+          else gv = GKe[(s * TRH + rta) * 64 + lane];
+          if (k != AE_NONE) v += gv * sX[k];
+        }
       }
       v += __shfl_xor(v, 16, 64);
       v += __shfl_xor(v, 32, 64);
@@ -446,6 +482,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
       const int klen = L.sTab[rta];
       const unsigned char* kt = L.sTab + TRH + rta * 4 * KSMAX;
       double v = 0.0;
+#pragma unroll 8
       for (int idx = 0; idx < klen; ++idx) {
         const int k = kt[idx];
         v += GKe[row + NH * k] * sX[k];
@@ -467,26 +504,6 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         const int a = 4 * s + g;
         mic[dd][s] = (c < NPV && a < NPV) ? L.sMiC[c + NPV * a + NPV * NPV * dd] : 0.0;
       }
-  }
-  // ---- GK's A fragments for every RHS tile (row 16 rta + l%16, column of
-  // k-step s), in registers when they fit
-  constexpr int KSR = S::GKREG ? S::KSP : 1;
-  double gkf[KSR][TRH];
-  int kend[TRH];
-#pragma unroll
-  for (int rta = 0; rta < TRH; ++rta) kend[rta] = (__builtin_amdgcn_readfirstlane(L.sTab[rta]) + 3) / 4;
-  if constexpr (S::GKREG) {
-    if (sparse) {
-#pragma unroll
-      for (int s = 0; s < KSR; ++s)
-#pragma unroll
-        for (int rta = 0; rta < TRH; ++rta) {
-          const int k = L.sTab[TRH + rta * 4 * KSMAX + 4 * s + g];
-          const int row = 16 * rta + c;
-          if constexpr (TILE) gkf[s][rta] = s < kend[rta] ? GKe[(s * TRH + rta) * 64 + lane] : 0.0;
-          else gkf[s][rta] = (s < kend[rta] && k != AE_NONE && row < NH) ? GKe[row + NH * k] : 0.0;
-        }
-    }
   }
   __syncthreads();                            // sX (du) is reused below
 
