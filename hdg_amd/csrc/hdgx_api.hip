@@ -50,7 +50,7 @@ uint64_t fnv(uint64_t h, const void* p, size_t n) {
 // table in it, and a content hash (FNV-1a over the sizes and every table).
 struct MasterImage {
   std::vector<char> host;
-  size_t off[13];
+  size_t off[14];
   uint64_t hash;
 };
 
@@ -87,6 +87,35 @@ std::vector<unsigned char> ae_table(int npv, int npf, int nfe, int ncu, int nc, 
   return t;
 }
 
+// B fragments of the quadrature's BD product (DevMaster::bdv, BdShape in
+// hdgx_shapes.h): SV(r', (jj, ii)) over the volume rows r' < ngv (nd+1)
+// (shapvgdotshapvl, util/preprocess.m:23-37, as volint.m:262 contracts it) and
+// the face points r' = ngv (nd+1) + j ngf + g (shapfgdotshapfc(la, lb, g) with
+// la, lb the positions of jj, ii on face j, faceint.m:316,326, when both lie on
+// it), laid out as 64-lane vectors per (row tile R of pairs jj + npv ii, k-step
+// s): lane l = SV(4 s + l / 16, 16 R + l % 16), zero past either size.
+std::vector<double> bd_table(const hdgx_master* m, const std::vector<int32_t>& loc) {
+  const int npv = m->npv, npf = m->npf, nfe = m->nfe, ngv = m->ngv, ngf = m->ngf, nd = m->nd;
+  const int nvv = npv * npv, rtb = (nvv + 15) / 16, nr = ngv * (nd + 1), kt = nr + nfe * ngf, ksb = (kt + 3) / 4;
+  std::vector<double> t((size_t)rtb * ksb * 64, 0.0);
+  for (int R = 0; R < rtb; ++R)
+    for (int s = 0; s < ksb; ++s)
+      for (int l = 0; l < 64; ++l) {
+        const int r = 4 * s + l / 16, col = 16 * R + l % 16;
+        if (r >= kt || col >= nvv) continue;
+        double v = 0.0;
+        if (r < nr) {
+          v = m->shapvgdotshapvl[col + (size_t)nvv * r];
+        } else {
+          const int tt = r - nr, j = tt / ngf, g = tt % ngf, jj = col % npv, ii = col / npv;
+          const int la = loc[jj + (size_t)npv * j], lb = loc[ii + (size_t)npv * j];
+          if (la >= 0 && lb >= 0) v = m->shapfgdotshapfc[la + npf * lb + (size_t)npf * npf * g];
+        }
+        t[((size_t)R * ksb + s) * 64 + l] = v;
+      }
+  return t;
+}
+
 int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   if (!m || !m->shapvt || !m->shapvg || !m->shapvgdotshapvl || !m->shapft || !m->shapfg || !m->shapfgdotshapfc ||
       !m->shapmv || !m->shapmf || !m->perm || !m->permgeom)
@@ -118,15 +147,21 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   const int ncu = nd + 2;
   const std::vector<unsigned char> aek_u = ae_table(npv, npf, nfe, ncu, ncu, m->perm);
   const std::vector<unsigned char> aek_uq = ae_table(npv, npf, nfe, ncu, ncu * (nd + 1), m->perm);
-  const size_t nd64 = n_vt + n_vg + n_vv + n_ft + n_fg + n_ff + n_mv + n_mf;
-  img->host.assign(nd64 * 8 + (2 * n_perm + n_loc) * 4 + aek_u.size() + aek_uq.size(), 0);
-  size_t at = 0;
+  const std::vector<double> bdv = bd_table(m, loc);
+  // every table starts on a 16-byte boundary
+  img->host.clear();
   int k = 0;
-  auto put = [&](const void* src, size_t n) { std::memcpy(img->host.data() + at, src, n); img->off[k++] = at; at += n; };
+  auto put = [&](const void* src, size_t n) {
+    const size_t at = (img->host.size() + 15) / 16 * 16;
+    img->host.resize(at + n, 0);
+    std::memcpy(img->host.data() + at, src, n);
+    img->off[k++] = at;
+  };
   put(m->shapvt, n_vt * 8); put(m->shapvg, n_vg * 8); put(m->shapvgdotshapvl, n_vv * 8); put(m->shapft, n_ft * 8);
   put(m->shapfg, n_fg * 8); put(m->shapfgdotshapfc, n_ff * 8); put(m->shapmv, n_mv * 8); put(m->shapmf, n_mf * 8);
   put(m->perm, n_perm * 4); put(m->permgeom, n_perm * 4); put(loc.data(), n_loc * 4);
   put(aek_u.data(), aek_u.size()); put(aek_uq.data(), aek_uq.size());
+  put(bdv.data(), bdv.size() * 8);
   return HDGX_OK;
 }
 
@@ -138,6 +173,7 @@ void bind_master(const hdgx_master* m, const MasterImage& img, char* base, Maste
   mb->tab.shapmv = (const double*)dp(6); mb->tab.shapmf = (const double*)dp(7);
   mb->tab.perm = (const int32_t*)dp(8); mb->tab.permgeom = (const int32_t*)dp(9); mb->tab.loc = (const int32_t*)dp(10);
   mb->tab.aek[0] = (const unsigned char*)dp(11); mb->tab.aek[1] = (const unsigned char*)dp(12);
+  mb->tab.bdv = (const double*)dp(13);
   mb->nd = m->nd; mb->porder = m->porder; mb->npv = m->npv; mb->npf = m->npf; mb->nfe = m->nfe;
   mb->ngv = m->ngv; mb->ngf = m->ngf;
 }
@@ -1130,6 +1166,14 @@ int hdgx_plan_create(hdgx_ctx* ctx, int nseg, const hdgx_master* const* masters,
     if ((rc = get_master(ctx, masters[s0], &mb))) return rc;
     for (int s = 0; s < nseg; ++s)
       if (pof[s] == p && masters[s]->npv != mb->npv) return fail(ctx, HDGX_ERR_ARG, "plan: masters of one order differ");
+    rc = with_cfg(P->phys, P->nd, p, [&](auto cfg) {
+      using Cc = decltype(cfg);
+      if (mb->npv != Cc::NPV || mb->npf != Cc::NPF || mb->ngv != Cc::NGV || mb->ngf != Cc::NGF)
+        return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: master quadrature does not match the compiled configuration");
+      return (int)HDGX_OK;
+    });
+    if (rc < 0) return fail(ctx, HDGX_ERR_UNSUPPORTED, "plan: no kernel for this order");
+    if (rc) return rc;
     P->tab[p] = mb->tab;
     P->perm[p] = mb->tab.perm;
   }

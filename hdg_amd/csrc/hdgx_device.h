@@ -10,6 +10,14 @@
 
 namespace hdgx {
 
+// 16x16x4 MFMA operands and accumulators: A is row l%16, k l/16 and B is
+// k l/16, column l%16 (lane l); an FP64 accumulator's register i holds row
+// 4i + l/16, column l%16.
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+HDGX_DEV d4 mfma64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+
 // ------------------------------------------------------------- dual numbers
 template <int N>
 struct Dual {

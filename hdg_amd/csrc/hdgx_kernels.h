@@ -39,6 +39,12 @@ struct DevMaster {
   const double* sff;              // shapfgdotshapfc (npf*npf, ngf, nd)
   const double* shapmv;           // (ngv, npv, nd+1)
   const double* shapmf;           // (ngf, npf, nd)
+  // B fragments of the quadrature's BD product (BdShape): per row tile R of
+  // volume node pairs (jj, ii) (row jj + npv ii) and k-step s, 64 doubles,
+  // lane l = SV(row 4 s + l / 16, pair 16 R + l % 16); SV rows r < ngv (nd+1)
+  // are shapvgdotshapvl, rows ngv (nd+1) + j ngf + g shapfgdotshapfc(loc(jj, j),
+  // loc(ii, j), g) where both nodes lie on face j, else 0
+  const double* bdv;
   const int32_t* perm;            // (npf, nfe)
   const int32_t* permgeom;        // (npf, nfe)
   const int32_t* loc;             // (npv, nfe): position of volume node v on face j, or -1

@@ -11,6 +11,17 @@
 
 namespace hdgx {
 
+// The quadrature's BD product: BD^T rows (c, k) (NCC = ncu nc, CTB tiles),
+// columns the volume node pairs (jj, ii) (NVV = npv^2, RTB tiles), k over the
+// NR volume rows and the nfe ngf face points (KT, KSB k-steps).  The host's
+// table of B fragments (bd_table) uses the same sizes.
+template <class C>
+struct BdShape {
+  static constexpr int NVV = C::NPV * C::NPV, RTB = (NVV + 15) / 16;
+  static constexpr int KT = C::NR + C::NFE * C::NGF, KSB = (KT + 3) / 4;
+  static constexpr int NCC = C::NCU * C::NC, CTB = (NCC + 15) / 16;
+};
+
 template <class C>
 struct S64Shape {
   static constexpr int NU = C::NU, NH = C::NH, NPV = C::NPV, NCU = C::NCU, ND = C::ND, NC = C::NC;

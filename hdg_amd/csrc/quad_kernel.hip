@@ -292,9 +292,10 @@ struct QuadLds {
   double sUf[NFG * NC];
   double sUHf[NFG * NCU];
   double sWrk[NR * NCU];
-  double sWrl[RO ? 1 : NR * NCU * NC];
+  // the B rows of BD's product (BdShape): volume rows wrl (r < NR), then the
+  // face rows fhat_udg (NR + t), each row's NCU*NC columns contiguous
+  double sWL[RO ? 1 : BdShape<C>::KT * BdShape<C>::NCC + 16];
   double sFH[NFG * NCU];
-  double sFHu[RO ? 1 : NFG * NCU * NC];
   double sFHh[RO ? 1 : NFG * NCU * NCU];
   double sGH[NFG * NCU];
   double sGHu[RO ? 1 : NFG * NCU * NC];
@@ -314,6 +315,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
   constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
+  constexpr int NCC = NCU * NC;          // BD columns per volume node pair: (c, k)
   constexpr int NDIRF = NC + NCU;        // face derivative directions: UDG components, then UH
   constexpr bool UQ = C::UQ;
 
@@ -342,9 +344,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   double* const sUf = L.sUf;
   double* const sUHf = L.sUHf;
   double* const sWrk = L.sWrk;
-  double* const sWrl = L.sWrl;
+  double* const sWL = L.sWL;
   double* const sFH = L.sFH;
-  double* const sFHu = L.sFHu;
   double* const sFHh = L.sFHh;
   double* const sGH = L.sGH;
   double* const sGHu = L.sGHu;
@@ -511,13 +512,13 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           for (int b = 0; b < ND; ++b) Xx[a][b] = sXx[g * ND * ND + a * ND + b];
   #pragma unroll
         for (int c = 0; c < NCU; ++c) {
-          sWrl[g + NR * (c + NCU * k)] = (tdep && k == c) ? ap.fc_u * jac : 0.0;
+          sWL[(c + NCU * k) + NCC * g] = (tdep && k == c) ? ap.fc_u * jac : 0.0;
   #pragma unroll
           for (int i = 0; i < ND; ++i) {
             double v = 0.0;
   #pragma unroll
             for (int j = 0; j < ND; ++j) v += f[c][j].d[0] * Xx[j][i];
-            sWrl[g + NGV * (i + 1) + NR * (c + NCU * k)] = -v;
+            sWL[(c + NCU * k) + NCC * (g + NGV * (i + 1))] = -v;
           }
         }
         if (k == 0) {
@@ -545,7 +546,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
       for (int c = 0; c < NCU; ++c) {
         if (k < NC) {
-          sFHu[t + NFG * (c + NCU * k)] = fa[c].d[0] * jac;
+          sWL[(c + NCU * k) + NCC * (NR + t)] = fa[c].d[0] * jac;
           sGHu[t + NFG * (c + NCU * k)] = fb[c].d[0] * jac;
         } else {
           sFHh[t + NFG * (c + NCU * (k - NC))] = fa[c].d[0] * jac;
@@ -592,45 +593,38 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 
   if constexpr (!RO) {
     // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
-    // faceint.m:316,326), one column (c, ii, k) per task
+    // faceint.m:316,326) as one FP64 MFMA product per element:
+    //   BD(jj, c, ii, k) = sum_r' WL(r', (c, k)) SV(r', (jj, ii)),
+    // r' over the volume rows r < NR (WL = wrl, SV = shapvgdotshapvl) and the
+    // face points NR + t (WL = the fhat Jacobian, SV = shapfgdotshapfc of
+    // (jj, ii) where both lie on the point's face, else 0).  SV is the master's
+    // B-fragment table (T.bdv); WL^T's A fragments come from LDS.  Each
+    // wavefront takes row tiles R of (jj, ii) and keeps their SV fragments in
+    // registers across the column tiles of (c, k).
+    using BS = BdShape<C>;
+    constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
     double* BD = A.BD + e * (int64_t)NU * C::NCOL;
-    for (int col = tid; col < NCU * NPV * NC; col += NT) {
-      const int c = col % NCU, ii = (col / NCU) % NPV, k = col / (NCU * NPV);
-      const double* wl = sWrl + NR * (c + NCU * k);
-      double acc[NPV];
+    const int wv = tid >> 6, ln = tid & 63, gq = ln >> 4, n = ln & 15;
+    for (int R = wv; R < BS::RTB; R += NT / 64) {
+      double b[KSB];
   #pragma unroll
-      for (int jj = 0; jj < NPV; ++jj) acc[jj] = 0.0;
-      double sv[NGV];
+      for (int s = 0; s < KSB; ++s) b[s] = T.bdv[(R * KSB + s) * 64 + ln];
+      const int col = 16 * R + n, jj = col % NPV, ii = col / NPV;
   #pragma unroll
-      for (int g = 0; g < NGV; ++g) sv[g] = sShV[g + NGV * ii];
+      for (int Cc = 0; Cc < BS::CTB; ++Cc) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
   #pragma unroll
-      for (int d = 0; d <= ND; ++d)
-  #pragma unroll
-        for (int g = 0; g < NGV; ++g) {
-          const int r = g + NGV * d;
-          const double tv = sv[g] * wl[r];
-  #pragma unroll
-          for (int jj = 0; jj < NPV; ++jj) acc[jj] += uload(T.shapvg, jj + NPV * r) * tv;
+        for (int s = 0; s < KSB; ++s) {
+          const int r = 4 * s + gq;
+          const double a = r < KT ? sWL[16 * Cc + n + NCC * r] : 0.0;   // rows past NCC only feed unused rows
+          acc = mfma64(a, b[s], acc);
         }
   #pragma unroll
-      for (int j = 0; j < NFE; ++j) {
-        const int lb = sLoc[ii + NPV * j];
-        if (lb >= 0) {
-          const double* fu = sFHu + j * NGF + NFG * (c + NCU * k);
-  #pragma unroll
-          for (int jj = 0; jj < NPV; ++jj) {
-            const int la = uload(T.loc, jj + NPV * j);
-            if (la >= 0) {
-              double v = 0.0;
-  #pragma unroll
-              for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * lb + NPF * NPF * g] * fu[g];
-              acc[jj] += v;
-            }
-          }
+        for (int i = 0; i < 4; ++i) {
+          const int cc = 16 * Cc + 4 * i + gq, c = cc % NCU, k = cc / NCU;
+          if (cc < NCC && col < NVV) BD[jj + NPV * (c + NCU * (ii + NPV * k))] = acc[i];
         }
       }
-  #pragma unroll
-      for (int jj = 0; jj < NPV; ++jj) BD[jj + NPV * col] = acc[jj];
     }
   }
 

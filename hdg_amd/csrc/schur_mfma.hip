@@ -23,11 +23,6 @@
 
 namespace hdgx {
 
-typedef double d4 __attribute__((ext_vector_type(4)));
-typedef float f4 __attribute__((ext_vector_type(4)));
-
-HDGX_DEV d4 mfma64(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-
 // the 16x16x4 MFMA of the elimination's element type.  Both forms take the A
 // operand as row l%16, k l/16 and B as k l/16, column l%16 (lane l = 16g + c);
 // their accumulators differ: register i holds row 4i + g (FP64) or row
