@@ -254,6 +254,44 @@ __global__ void f2f_kernel(int nd, int64_t ne, int64_t nf, const int32_t* __rest
   }
 }
 
+// CBSR pattern of the face system (mesh/mkmesh/block_crs.m via
+// elcon2entcon.m on t2f'): row i is face i, then the faces sharing an element
+// with it, ascending.  Two simplices share at most one face, so the f2f slots
+// other than the face itself are distinct and the row is f2f's valid slots.
+// One thread per face: the row length, then (after the scan) the row.
+__global__ void bcrs_count_kernel(int nbf, int64_t nf, const int32_t* __restrict__ f2f, int32_t* cnt, int* err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int32_t* row = f2f + (int64_t)nbf * i;
+  int n = 1;
+  if (row[0] != i) atomicOr(err, 1);
+  for (int k = 1; k < nbf; ++k) {
+    const int32_t v = row[k];
+    if (v >= nf || v == i) atomicOr(err, 2);
+    n += v >= 0;
+  }
+  cnt[i] = n;
+}
+
+__global__ void bcrs_fill_kernel(int nbf, int64_t nf, const int32_t* __restrict__ f2f,
+                                 const int32_t* __restrict__ rowpts, int32_t* colind) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nf) return;
+  const int32_t* row = f2f + (int64_t)nbf * i;
+  int32_t v[8];                                    // nbf - 1 <= 6 (nd <= 3)
+  int n = 0;
+  for (int k = 1; k < nbf; ++k)
+    if (row[k] >= 0) {
+      // insertion into the ascending list
+      int at = n++;
+      while (at > 0 && v[at - 1] > row[k]) { v[at] = v[at - 1]; --at; }
+      v[at] = row[k];
+    }
+  int32_t* out = colind + rowpts[i];
+  out[0] = (int32_t)i;
+  for (int k = 0; k < n; ++k) out[1 + k] = v[k];
+}
+
 // the context's mesh scratch, grown on demand and kept for later calls
 int ensure_mws(hdgx_ctx* ctx, size_t bytes) {
   if (ctx->mws_bytes >= bytes) return HDGX_OK;
@@ -366,5 +404,49 @@ int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t ne, int64_t nf, const int32_t* 
   HIP_TRY(ctx, hipStreamSynchronize(s));
   if (herr & 16) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: an element id of f outside 0..ne-1");
   if (herr) return fail(ctx, HDGX_ERR_ARG, "mesh_f2f: a face missing from its element's t2f");
+  return HDGX_OK;
+}
+
+int hdgx_mesh_block_crs(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f2f, int32_t* rowpts, int32_t* colind,
+                        int64_t* nblks) {
+  if (!ctx) return HDGX_ERR_ARG;
+  if ((nd != 2 && nd != 3) || nf < 0 || !nblks) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: sizes");
+  if (nf > INT32_MAX / (2 * nd + 1)) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: too many faces");
+  *nblks = 0;
+  if (!rowpts) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: null array");
+  HIP_TRY(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = ctx->stream;
+  if (nf == 0) {
+    HIP_TRY(ctx, hipMemsetAsync(rowpts, 0, sizeof(int32_t), s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return HDGX_OK;
+  }
+  if (!f2f || !colind) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: null array");
+  const int nbf = 2 * nd + 1;
+  const int64_t nb = (nf + SCAN_BLOCK - 1) / SCAN_BLOCK;
+  // err | cnt (nf) | bsum (nb)
+  int rc = ensure_mws(ctx, 16 + (size_t)nf * 4 + (size_t)nb * 4);
+  if (rc) return rc;
+  char* p = static_cast<char*>(ctx->mws);
+  auto* err = reinterpret_cast<int32_t*>(p);
+  auto* cnt = reinterpret_cast<int32_t*>(p + 16);
+  auto* bsum = cnt + nf;
+  HIP_TRY(ctx, hipMemsetAsync(err, 0, sizeof(int32_t), s));
+  hipLaunchKernelGGL(bcrs_count_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nbf, nf, f2f, cnt, err);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, nf, cnt, rowpts, bsum);
+  hipLaunchKernelGGL(scan_sums_kernel, dim3(1), dim3(SCAN_THREADS), 0, s, nb, bsum, rowpts + nf);
+  hipLaunchKernelGGL(add_offsets_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nf, rowpts, bsum);
+  HIP_TRY(ctx, hipGetLastError());
+  // the row lengths are checked before colind is written
+  int32_t h2[2] = {0, 0};
+  HIP_TRY(ctx, hipMemcpyAsync(&h2[0], err, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipMemcpyAsync(&h2[1], rowpts + nf, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (h2[0] & 1) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: f2f(0, i) is not face i");
+  if (h2[0] & 2) return fail(ctx, HDGX_ERR_ARG, "mesh_block_crs: an f2f entry at or past nf, or a face its own neighbour");
+  hipLaunchKernelGGL(bcrs_fill_kernel, dim3(grid_of(nf, 256)), dim3(256), 0, s, nbf, nf, f2f, rowpts, colind);
+  HIP_TRY(ctx, hipGetLastError());
+  HIP_TRY(ctx, hipStreamSynchronize(s));
+  *nblks = h2[1];
   return HDGX_OK;
 }

@@ -413,6 +413,9 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
   };
   double gkc[NBT][KSF], gkn[NBT][KSF];
   load_gk(0, gkc);
+  // X and the AE blocks are formed transposed (operands swapped), so register
+  // i of lane (g, c) holds row 16 t + c, column 16 tb + 4 i + g of its tile:
+  // sixteen lanes store sixteen consecutive rows of a column
   double hv[NBT][NBT][4];
 #pragma unroll
   for (int sr = 0; sr < NBT; ++sr)
@@ -420,11 +423,11 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
     for (int tb = 0; tb < NBT; ++tb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int rr = 16 * sr + 4 * i + g, r = 16 * tb + c;
+        const int rr = 16 * sr + c, r = 16 * tb + 4 * i + g;
         hv[sr][tb][i] = (rr < NB && r < NB) ? Hb[rr + NB * r] : 0.0;
       }
 
-  // ---- X = D^-1(:, u rows of fc) (-F_fc), face-local k
+  // ---- X = D^-1(:, u rows of fc) (-F_fc), face-local k (X^T = (-F_fc)^T D^-T)
   d4 x[RT][NBT];
 #pragma unroll
   for (int rt = 0; rt < RT; ++rt)
@@ -445,7 +448,7 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
 #pragma unroll
     for (int rt = 0; rt < RT; ++rt)
 #pragma unroll
-      for (int tb = 0; tb < NBT; ++tb) x[rt][tb] = mfma64(a[rt], b[tb], x[rt][tb]);
+      for (int tb = 0; tb < NBT; ++tb) x[rt][tb] = mfma64(b[tb], a[rt], x[rt][tb]);
   }
 
   // ---- DUDG_DUH columns of face fc, and X to LDS
@@ -457,7 +460,7 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
     for (int tb = 0; tb < NBT; ++tb)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const int row = 16 * rt + 4 * i + g, r = 16 * tb + c;
+        const int row = 16 * rt + c, r = 16 * tb + 4 * i + g;
         const double v = x[rt][tb][i];
         L.sX[row + XS * r] = v;
         if (row < LDO && r < NB) {
@@ -487,14 +490,14 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
 #pragma unroll
         for (int tb = 0; tb < NBT; ++tb) {
           const double b = q < NB ? L.sX[uq + XS * (16 * tb + c)] : 0.0;
-          ae[tb] = mfma64(a, b, ae[tb]);
+          ae[tb] = mfma64(b, a, ae[tb]);
         }
       }
 #pragma unroll
       for (int tb = 0; tb < NBT; ++tb)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-          const int rr = 16 * sr + 4 * i + g, r = 16 * tb + c;
+          const int rr = 16 * sr + c, r = 16 * tb + 4 * i + g;
           if (rr < NB && r < NB) {
             const double v = ae[tb][i];
             if (!isfinite(v)) st |= 4;

@@ -76,6 +76,7 @@ def lib():
         L.hdgx_mesh_faces.argtypes = [vp, i32, i64, i32, vp, vp, vp, vp]
         L.hdgx_mesh_elcon.argtypes = [vp, i32, i32, i32, i32, i64, i64, vp, vp, vp, vp, vp]
         L.hdgx_mesh_f2f.argtypes = [vp, i32, i64, i64, vp, vp, vp, vp]
+        L.hdgx_mesh_block_crs.argtypes = [vp, i32, i64, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -87,7 +88,7 @@ EXPORTS = ["hdgx_create", "hdgx_destroy", "hdgx_set_stream", "hdgx_set_precision
            "hdgx_plan_create", "hdgx_plan_launch", "hdgx_plan_capture", "hdgx_plan_destroy",
            "hdgx_trace_assemble", "hdgx_trace_matvec", "hdgx_bilu0_factor", "hdgx_bilu0_apply", "hdgx_gmres",
            "hdgx_trace_gather", "hdgx_trace_scatter", "hdgx_color_faces", "hdgx_local_update", "hdgx_local_solve", "hdgx_newton_update", "hdgx_norm2",
-           "hdgx_mesh_faces", "hdgx_mesh_elcon", "hdgx_mesh_f2f"]
+           "hdgx_mesh_faces", "hdgx_mesh_elcon", "hdgx_mesh_f2f", "hdgx_mesh_block_crs"]
 
 
 def _torch():
@@ -400,6 +401,18 @@ class Context:
         f2f = Dev.empty((2 * nd + 1, nf), "int32")
         self._check(lib().hdgx_mesh_f2f(self.h, nd, ne, nf, f.ptr, t2f.ptr, fconn.ptr, f2f.ptr))
         return fconn, f2f
+
+    def mesh_block_crs(self, f2f):
+        """CBSR pattern of the face system on the device (block_crs.m with
+        elcon2entcon.m on t2f'): f2f (2 nd + 1, nf) as mesh_f2f gives it.
+        Returns (rowpts (nf+1,) from 0, colind (nblks,) 0-based) Dev arrays."""
+        nbf, nf = f2f.shape
+        nd = (nbf - 1) // 2
+        rowpts = Dev.empty((nf + 1,), "int32")
+        colind = Dev.empty((max(nf * nbf, 1),), "int32")
+        nblks = ctypes.c_int64(0)
+        self._check(lib().hdgx_mesh_block_crs(self.h, nd, nf, f2f.ptr, rowpts.ptr, colind.ptr, ctypes.byref(nblks)))
+        return rowpts, Dev(colind.t[:nblks.value], (nblks.value,))
 
     def getq(self, master, dgnodes, udg, uh, sh=None, fc_q=1.0, ncu=None, out=None, inplace=False, shapnv=None):
         """QDG = getq(master, mesh, UDG, UH, SH, fc_q) (kernel/getq.m:1-129) for the

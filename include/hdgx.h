@@ -411,6 +411,18 @@ int hdgx_mesh_elcon(hdgx_ctx* ctx, int nd, int npv, int ncd, int npf, int64_t ne
 int hdgx_mesh_f2f(hdgx_ctx* ctx, int nd, int64_t ne, int64_t nf, const int32_t* f, const int32_t* t2f, int32_t* fconn,
                   int32_t* f2f);
 
+/* CBSR pattern of the face-block trace system on the device
+ * (mesh/mkmesh/block_crs.m:45-62 with elcon2entcon.m on t2f', 0-based): row i
+ * is face i, then the other faces sharing an element with it, ascending.
+ *   f2f (2nd+1, nf) int32 as hdgx_mesh_f2f gives it; rowpts (nf+1) int32 out,
+ *   offsets from 0 (cbsr_rowpts); colind int32 out (cbsr_colind, 0-based face
+ *   ids), room for nf*(2nd+1) entries; *nblks = rowpts(nf) (cbsr_nblks).
+ * Device arrays; synchronises the context's stream.  HDGX_ERR_ARG when
+ * f2f(0, i) is not i or another entry is i or at least nf (colind is then not
+ * written). */
+int hdgx_mesh_block_crs(hdgx_ctx* ctx, int nd, int64_t nf, const int32_t* f2f, int32_t* rowpts, int32_t* colind,
+                        int64_t* nblks);
+
 /* RH(:, elcon(:, e)) += Rh(:, :, e) over all elements: the global trace
  * residual of kernel/hdg_residual.m:41-52 (sparse(il, 1, RH)), summed per face
  * from its one or two elements (no atomics, the same bits every run).

@@ -1,7 +1,8 @@
 """Mesh connectivity on the device (hdgx_mesh_faces, hdgx_mesh_elcon,
 hdgx_mesh_f2f) equals the host construction of hdg_amd/mesh.py and
-hdg_amd/trace.py (mesh/mkmesh/mkt2f.m, elconnectivities.m:147-212, mkf2f.m)
-integer for integer: square and cube meshes,
+hdg_amd/trace.py (mesh/mkmesh/mkt2f.m, elconnectivities.m:147-212, mkf2f.m),
+and the CBSR pattern of hdgx_mesh_block_crs equals the oracle's elcon2entcon
+restatement (block_crs.m), integer for integer: square and cube meshes,
 jittered and re-oriented, one element, no element, and the error paths
 (a face of three elements, a vertex id out of range, trace nodes that do not
 meet)."""
@@ -20,6 +21,7 @@ from hdg_amd.runtime import HdgxError  # noqa: E402
 from hdg_amd.trace import face_connectivity, mkf2f  # noqa: E402
 from hdg_amd.runtime import Dev  # noqa: E402
 from tests.test_mkt2f_bruteforce import MESHES, mkt2f_loop  # noqa: E402
+from oracle import trace_oracle as TO  # noqa: E402
 
 
 @pytest.fixture(scope="module")
@@ -44,6 +46,10 @@ def _check(ctx, p, t, master):
     fconn, f2f = ctx.mesh_f2f(f, t2f, nf)
     assert np.array_equal(fconn.numpy(), face_connectivity(ms))
     assert np.array_equal(f2f.numpy(), mkf2f(ms))
+    rowpts, colind = ctx.mesh_block_crs(f2f)
+    r_ref, c_ref = TO.entcon(t2f_ref)
+    assert np.array_equal(rowpts.numpy(), r_ref)
+    assert np.array_equal(colind.numpy(), c_ref)
 
 
 @pytest.mark.parametrize("nx,ny,porder,amp", [(7, 5, 3, 0.2), (64, 64, 2, 0.0), (200, 150, 4, 0.1)])
@@ -126,3 +132,22 @@ def test_errors(ctx):
     dg[:, :, 1] += 1e-6
     with pytest.raises(HdgxError, match="do not match"):
         ctx.mesh_elcon(dg, m.perm, f, t2f, nf)
+
+
+def test_block_crs_errors(ctx):
+    """A face table whose slot 0 is not the face, or with an entry past nf,
+    raises before colind is written; no faces gives rowpts = [0]."""
+    print("[gpu] device block_crs error paths", flush=True)
+    p, t = squaremesh(4, 4)
+    f, t2f, nf = ctx.mesh_faces(t)
+    _, f2f = ctx.mesh_f2f(f, t2f, nf)
+    bad = f2f.numpy().copy()
+    bad[2, 5] = nf
+    with pytest.raises(HdgxError, match="block_crs"):
+        ctx.mesh_block_crs(Dev.from_numpy(bad, "int32"))
+    bad = f2f.numpy().copy()
+    bad[0, 3] = 4
+    with pytest.raises(HdgxError, match="block_crs"):
+        ctx.mesh_block_crs(Dev.from_numpy(bad, "int32"))
+    rowpts, colind = ctx.mesh_block_crs(Dev.empty((5, 0), "int32"))
+    assert rowpts.numpy().tolist() == [0] and colind.shape == (0,)
