@@ -142,6 +142,11 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
       if (v < 0 || v >= npv) return fail(ctx, HDGX_ERR_ARG, "master: perm out of range");
       loc[v + (size_t)npv * j] = a;
     }
+  // a face lists each of its nodes once: loc(perm(a, j), j) = a (the
+  // quadrature kernels' face-block keys rely on it)
+  for (int j = 0; j < nfe; ++j)
+    for (int a = 0; a < npf; ++a)
+      if (loc[m->perm[a + npf * j] + (size_t)npv * j] != a) return fail(ctx, HDGX_ERR_ARG, "master: perm repeats a node on a face");
   for (size_t i = 0; i < n_perm; ++i)
     if (m->permgeom[i] < 0 || m->permgeom[i] >= npv) return fail(ctx, HDGX_ERR_ARG, "master: permgeom out of range");
   const int ncu = nd + 2;

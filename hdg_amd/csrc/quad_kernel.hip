@@ -264,6 +264,87 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
   for (int c = 0; c < NCU; ++c) fb[c] = fh[c];
 }
 
+// Gather keys of F, GK and H in the compact workspace layouts (WsLayout in
+// hdgx_shapes.h), fixed by the configuration alone and built at compile time:
+// every entry is a sum over the face points g of shapfgdotshapfc(la, b, g)
+// times a face-flux Jacobian entry X(j NGF + g, (c, k)) (faceint.m:316-359),
+// so a key carries the two LDS offsets and the output loops do no index
+// arithmetic.  -1: a structural zero.
+//   face blocks (F, GK, H) and fragment-tile H: so | xo << 8, with so = la +
+//     NPF b the pair's offset and xo = j NGF + NFG (c + NCU k) the column's;
+//   fragment-tile F: vj | b << 8 | xo << 12, la = loc(vj) read at run time
+//     (vj = va + NPV j; a volume node off face j gives 0);
+//   fragment-tile GK: tp | j << 10 | b << 14 | c << 18, the column (va, k)
+//     read at run time from the AE k-step table at tp.
+template <class C, bool FB>
+struct FghKeys {
+  using S = S64Shape<C>;
+  using FS = FbShape<C>;
+  static constexpr int NPV = C::NPV, NPF = C::NPF, NCU = C::NCU, NDF = C::NDF, NH = C::NH, NU = C::NU;
+  static constexpr int NGF = C::NGF, NFG = C::NFE * C::NGF, NB = FS::NB;
+  static constexpr int NF = FB ? (int)FS::BLK : (NU / 4) * S::TRH * 64;
+  static constexpr int NG = FB ? (int)FS::BLK : S::KSP * S::TRH * 64;
+  static constexpr int NHV = NH / 4;
+  static constexpr int NHK = FB ? (int)FS::BLK : S::TRH * NHV * 64;
+  int32_t f[NF], gk[NG], h[NHK];
+};
+
+template <class C, bool FB>
+constexpr FghKeys<C, FB> make_fgh_keys() {
+  using K = FghKeys<C, FB>;
+  using S = S64Shape<C>;
+  constexpr int NPV = K::NPV, NPF = K::NPF, NCU = K::NCU, NDF = K::NDF, NH = K::NH, NB = K::NB;
+  constexpr int NGF = K::NGF, NFG = K::NFG;
+  K k{};
+  auto pair = [](int so, int xo) { return so | (xo << 8); };
+  if constexpr (FB) {
+    for (int idx = 0; idx < K::NF; ++idx) {
+      // F_f(q, r): q = a + NPF m (face node a of face f, component m), r = kk + NCU b
+      const int f = idx / (NB * NB), rem = idx % (NB * NB), q = rem % NB, r = rem / NB;
+      k.f[idx] = pair(q % NPF + NPF * (r / NCU), f * NGF + NFG * (q / NPF + NCU * (r % NCU)));
+    }
+    for (int idx = 0; idx < K::NG; ++idx) {
+      // GK_f(r, q): trace row r = c + NCU b, volume q = a + NPF kk
+      const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, q = rem / NB;
+      k.gk[idx] = pair(q % NPF + NPF * (r / NCU), f * NGF + NFG * (r % NCU + NCU * (q / NPF)));
+    }
+    for (int idx = 0; idx < K::NHK; ++idx) {
+      const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, r2 = rem / NB;
+      k.h[idx] = pair(r / NCU + NPF * (r2 / NCU), f * NGF + NFG * (r % NCU + NCU * (r2 % NCU)));
+    }
+  } else {
+    for (int idx = 0; idx < K::NF; ++idx) {
+      const int ln = idx & 63, rest = idx >> 6, tb = rest % S::TRH, ri = rest / S::TRH;
+      const int row = 4 * ri + (ln >> 4), jj = 16 * tb + (ln & 15);
+      if (jj >= NH) { k.f[idx] = -1; continue; }
+      const int va = row % NPV, c = row / NPV, kk = jj / NDF, bj = jj % NDF, j = bj / NPF, b = bj % NPF;
+      k.f[idx] = (va + NPV * j) | (b << 8) | ((j * NGF + NFG * (c + NCU * kk)) << 12);
+    }
+    for (int idx = 0; idx < K::NG; ++idx) {
+      const int ln = idx & 63, rest = idx >> 6, rta = rest % S::TRH, s = rest / S::TRH;
+      const int row = 16 * rta + (ln & 15);
+      if (row >= NH) { k.gk[idx] = -1; continue; }
+      const int tp = S::TRH + rta * 4 * S::KSMAX + 4 * s + (ln >> 4);
+      const int c = row % NCU, bj = row / NCU, j = bj / NPF, b = bj % NPF;
+      k.gk[idx] = tp | (j << 10) | (b << 14) | (c << 18);
+    }
+    for (int idx = 0; idx < K::NHK; ++idx) {
+      const int ln = idx & 63, rest = idx >> 6, rih = rest % K::NHV, tb = rest / K::NHV;
+      const int row = 4 * rih + (ln >> 4), jj = 16 * tb + (ln & 15);
+      if (jj >= NH) { k.h[idx] = -1; continue; }
+      const int c = row % NCU, aj = row / NCU, kk = jj / NDF, bj = jj % NDF;
+      const int j = aj / NPF, a = aj % NPF, j2 = bj / NPF, b = bj % NPF;
+      k.h[idx] = j == j2 ? pair(a + NPF * b, j * NGF + NFG * (c + NCU * kk)) : -1;
+    }
+  }
+  return k;
+}
+
+template <class C, bool FB>
+struct FghKeyTab {
+  static constexpr FghKeys<C, FB> tab = make_fgh_keys<C, FB>();
+};
+
 // LDS of one element's quadrature (one workgroup).  RO: residual only (no
 // Jacobian arrays).
 template <class C, bool RO = false>
@@ -591,43 +672,6 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   __syncthreads();
 
-  if constexpr (!RO) {
-    // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
-    // faceint.m:316,326) as one FP64 MFMA product per element:
-    //   BD(jj, c, ii, k) = sum_r' WL(r', (c, k)) SV(r', (jj, ii)),
-    // r' over the volume rows r < NR (WL = wrl, SV = shapvgdotshapvl) and the
-    // face points NR + t (WL = the fhat Jacobian, SV = shapfgdotshapfc of
-    // (jj, ii) where both lie on the point's face, else 0).  SV is the master's
-    // B-fragment table (T.bdv); WL^T's A fragments come from LDS.  Each
-    // wavefront takes row tiles R of (jj, ii) and keeps their SV fragments in
-    // registers across the column tiles of (c, k).
-    using BS = BdShape<C>;
-    constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
-    double* BD = A.BD + e * (int64_t)NU * C::NCOL;
-    const int wv = tid >> 6, ln = tid & 63, gq = ln >> 4, n = ln & 15;
-    for (int R = wv; R < BS::RTB; R += NT / 64) {
-      double b[KSB];
-  #pragma unroll
-      for (int s = 0; s < KSB; ++s) b[s] = T.bdv[(R * KSB + s) * 64 + ln];
-      const int col = 16 * R + n, jj = col % NPV, ii = col / NPV;
-  #pragma unroll
-      for (int Cc = 0; Cc < BS::CTB; ++Cc) {
-        d4 acc = {0.0, 0.0, 0.0, 0.0};
-  #pragma unroll
-        for (int s = 0; s < KSB; ++s) {
-          const int r = 4 * s + gq;
-          const double a = r < KT ? sWL[16 * Cc + n + NCC * r] : 0.0;   // rows past NCC only feed unused rows
-          acc = mfma64(a, b[s], acc);
-        }
-  #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int cc = 16 * Cc + 4 * i + gq, c = cc % NCU, k = cc / NCU;
-          if (cc < NCC && col < NVV) BD[jj + NPV * (c + NCU * (ii + NPV * k))] = acc[i];
-        }
-      }
-    }
-  }
-
   // ---- 3b. Ru = shapvg*wrk - sum_faces shapfg*FH (volint.m:258, faceint.m:302-330)
   double* Ru = A.Ru + e * NU;
   for (int idx = tid; idx < NU; idx += NT) {
@@ -654,6 +698,53 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     Rh[idx] = RO ? v : -v;
   }
   if constexpr (!RO) {
+    // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
+    // faceint.m:316,326) as one FP64 MFMA product per element:
+    //   BD(jj, c, ii, k) = sum_r' WL(r', (c, k)) SV(r', (jj, ii)),
+    // r' over the volume rows r < NR (WL = wrl, SV = shapvgdotshapvl) and the
+    // face points NR + t (WL = the fhat Jacobian, SV = shapfgdotshapfc of
+    // (jj, ii) where both lie on the point's face, else 0).  SV is the master's
+    // B-fragment table (T.bdv); WL^T's A fragments come from LDS.  Each
+    // wavefront takes row tiles R of (jj, ii) and keeps their SV fragments in
+    // registers across the column tiles of (c, k).
+    // the gather keys of F, GK and H (FghKeys): F's loaded before the BD
+    // products (whose MFMA hides their latency), GK's and H's one array ahead
+    constexpr bool FBK = FbShape<C>::USE;
+    using KT_ = FghKeys<C, FBK>;
+    constexpr int IF = TILE ? (KT_::NF + NT - 1) / NT : 1, IG = TILE ? (KT_::NG + NT - 1) / NT : 1;
+    constexpr int IH = TILE ? (KT_::NHK + NT - 1) / NT : 1;
+    auto load_keys = [&](const int32_t* key, int nv, int* k, int n) {
+  #pragma unroll
+      for (int it = 0; it < n; ++it) k[it] = tid + it * NT < nv ? key[tid + it * NT] : -1;
+    };
+    int kf[IF], kg[IG], kh[IH];
+    if constexpr (TILE) load_keys(FghKeyTab<C, FBK>::tab.f, KT_::NF, kf, IF);
+    using BS = BdShape<C>;
+    constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
+    double* BD = A.BD + e * (int64_t)NU * C::NCOL;
+    const int wv = tid >> 6, ln = tid & 63, gq = ln >> 4, n = ln & 15;
+    for (int R = wv; R < BS::RTB; R += NT / 64) {
+      double b[KSB];
+  #pragma unroll
+      for (int s = 0; s < KSB; ++s) b[s] = T.bdv[(R * KSB + s) * 64 + ln];
+      const int col = 16 * R + n, jj = col % NPV, ii = col / NPV;
+  #pragma unroll
+      for (int Cc = 0; Cc < BS::CTB; ++Cc) {
+        d4 acc = {0.0, 0.0, 0.0, 0.0};
+  #pragma unroll
+        for (int s = 0; s < KSB; ++s) {
+          const int r = 4 * s + gq;
+          const double a = r < KT ? sWL[16 * Cc + n + NCC * r] : 0.0;   // rows past NCC only feed unused rows
+          acc = mfma64(a, b[s], acc);
+        }
+  #pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int cc = 16 * Cc + 4 * i + gq, c = cc % NCU, k = cc / NCU;
+          if (cc < NCC && col < NVV) BD[jj + NPV * (c + NCU * (ii + NPV * k))] = acc[i];
+        }
+      }
+    }
+    if constexpr (TILE) load_keys(FghKeyTab<C, FBK>::tab.gk, KT_::NG, kg, IG);
     // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327); F(va, c, k, bj) is u row
     // va + NPV c, trace column k + NCU bj
     auto f_entry = [&](int va, int c, int k, int bj) {
@@ -665,24 +756,35 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sFHh[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
+    // a keyed entry: sum_g shapfgdotshapfc(so + NPF^2 g) X(xo + g) (FghKeys)
+    auto keyed = [&](int so, const double* X) {
+      double v = 0.0;
+  #pragma unroll
+      for (int g = 0; g < NGF; ++g) v += sSff[so + NPF * NPF * g] * X[g];
+      return v;
+    };
     if constexpr (TILE && FbShape<C>::USE) {
       // face blocks: F_f(q, r) = F(u row of q on face f, trace column f NB + r)
-      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+      constexpr int NV = (int)FbShape<C>::BLK;
       double* F = A.F + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int f = idx / (NB * NB), rem = idx - f * NB * NB, q = rem % NB, r = rem / NB;
-        const int va = T.perm[q % NPF + NPF * f];
-        F[idx] = f_entry(va, q / NPF, r % NCU, f * NPF + r / NCU);
+      load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+  #pragma unroll
+      for (int it = 0; it < IF; ++it) {
+        const int idx = tid + it * NT, kv = kf[it];
+        if (idx < NV) F[idx] = keyed(kv & 255, sFHh + (kv >> 8));
       }
     } else if constexpr (TILE) {
       // the elimination's fragment order (hdgx_shapes.h, WsLayout)
       using S = S64Shape<C>;
       constexpr int NV = (NU / 4) * S::TRH * 64;
       double* F = A.F + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int ln = idx & 63, rest = idx >> 6, tb = rest % S::TRH, ri = rest / S::TRH;
-        const int row = 4 * ri + (ln >> 4), jj = 16 * tb + (ln & 15);
-        F[idx] = jj < NH ? f_entry(row % NPV, row / NPV, jj / NDF, jj % NDF) : 0.0;
+      load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+  #pragma unroll
+      for (int it = 0; it < IF; ++it) {
+        const int idx = tid + it * NT, kv = kf[it];
+        if (idx >= NV) break;
+        const int la = kv >= 0 ? sLoc[kv & 255] : -1;
+        F[idx] = la >= 0 ? keyed(la + NPF * ((kv >> 8) & 15), sFHh + (kv >> 12)) : 0.0;
       }
     } else {
       double* F = A.F + e * (int64_t)NU * NH;
@@ -708,25 +810,33 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     };
     if constexpr (TILE && FbShape<C>::USE) {
       // face blocks: GK_f(r, q) = GK(trace row f NB + r, u column of q on face f)
-      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+      constexpr int NV = (int)FbShape<C>::BLK;
       double* GK = A.GK + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int f = idx / (NB * NB), rem = idx - f * NB * NB, r = rem % NB, q = rem / NB;
-        const int va = T.perm[q % NPF + NPF * f];
-        GK[idx] = gk_entry(r % NCU, f * NPF + r / NCU, va, q / NPF);
+  #pragma unroll
+      for (int it = 0; it < IG; ++it) {
+        const int idx = tid + it * NT, kv = kg[it];
+        if (idx < NV) GK[idx] = keyed(kv & 255, sGHu + (kv >> 8));
       }
     } else if constexpr (TILE) {
       // A fragments of the AE product over the face-sparse columns (the
-      // master's AE k-step table)
+      // master's AE k-step table gives each entry's column (va, k))
       using S = S64Shape<C>;
       constexpr int NV = S::KSP * S::TRH * 64;
       const unsigned char* tab = T.aek[UQ ? 1 : 0];
       double* GK = A.GK + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int ln = idx & 63, rest = idx >> 6, rta = rest % S::TRH, s = rest / S::TRH;
-        const int row = 16 * rta + (ln & 15);
-        const int kcol = tab[S::TRH + rta * 4 * S::KSMAX + 4 * s + (ln >> 4)];
-        GK[idx] = (row < NH && kcol != 255) ? gk_entry(row % NCU, row / NCU, kcol % NPV, kcol / NPV) : 0.0;
+  #pragma unroll
+      for (int it = 0; it < IG; ++it) {
+        const int idx = tid + it * NT, kv = kg[it];
+        if (idx >= NV) break;
+        const int kcol = kv >= 0 ? tab[kv & 1023] : 255;
+        double v = 0.0;
+        if (kcol != 255) {
+          const int j = (kv >> 10) & 15, b = (kv >> 14) & 15, c = kv >> 18;
+          const int va = kcol % NPV, k = kcol / NPV;
+          const int la = sLoc[va + NPV * j];
+          if (la >= 0) v = keyed(la + NPF * b, sGHu + j * NGF + NFG * (c + NCU * k));
+        }
+        GK[idx] = v;
       }
     } else {
       double* GK = A.GK + e * (int64_t)NH * C::NCOL;
@@ -748,22 +858,14 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE && FbShape<C>::USE) {
-      // face blocks: H_f(r, r') = H(f NB + r, f NB + r')
-      constexpr int NB = FbShape<C>::NB, NV = (int)FbShape<C>::BLK;
+    if constexpr (TILE) {
+      // face blocks H_f(r, r') = H(f NB + r, f NB + r'), or fragment tiles
+      constexpr int NV = KT_::NHK;
       double* H = A.H + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int f = idx / (NB * NB), rem = idx - f * NB * NB, r = rem % NB, r2 = rem / NB;
-        H[idx] = h_entry(r % NCU, f * NPF + r / NCU, r2 % NCU, f * NPF + r2 / NCU);
-      }
-    } else if constexpr (TILE) {
-      using S = S64Shape<C>;
-      constexpr int NHV = NH / 4, NV = S::TRH * NHV * 64;
-      double* H = A.H + e * (int64_t)NV;
-      for (int idx = tid; idx < NV; idx += NT) {
-        const int ln = idx & 63, rest = idx >> 6, rih = rest % NHV, tb = rest / NHV;
-        const int row = 4 * rih + (ln >> 4), jj = 16 * tb + (ln & 15);
-        H[idx] = jj < NH ? h_entry(row % NCU, row / NCU, jj / NDF, jj % NDF) : 0.0;
+  #pragma unroll
+      for (int it = 0; it < IH; ++it) {
+        const int idx = tid + it * NT, kv = kh[it];
+        if (idx < NV) H[idx] = kv >= 0 ? keyed(kv & 255, sGHh + (kv >> 8)) : 0.0;
       }
     } else {
       double* H = A.H + e * (int64_t)NH * NH;
