@@ -88,7 +88,7 @@ HDGX_DEV double face_normal(const double dpg[ND][ND > 1 ? ND - 1 : 1], double nl
 // Volume Gauss point g: jac and Xx from the geometry basis shapmv and the
 // element nodes X (npv, ncd), column-major with leading dimension NPV.
 template <class C>
-HDGX_DEV double vol_geom(const DevMaster& T, const double* X, int g, double Xx[C::ND][C::ND]) {
+HDGX_DEV double vol_geom(const double* shapmv, const double* X, int g, double Xx[C::ND][C::ND]) {
   constexpr int ND = C::ND, NPV = C::NPV, NGV = C::NGV;
   double J[ND][ND];
 #pragma unroll
@@ -97,7 +97,7 @@ HDGX_DEV double vol_geom(const DevMaster& T, const double* X, int g, double Xx[C
     for (int i = 0; i < ND; ++i) {
       double v = 0.0;
 #pragma unroll
-      for (int a = 0; a < NPV; ++a) v += T.shapmv[g + NGV * a + NGV * NPV * (1 + j)] * X[a + NPV * i];
+      for (int a = 0; a < NPV; ++a) v += shapmv[g + NGV * a + NGV * NPV * (1 + j)] * X[a + NPV * i];
       J[j][i] = v;   // dx_i / dxi_j
     }
   return adjugate<ND>(J, Xx);
@@ -105,7 +105,7 @@ HDGX_DEV double vol_geom(const DevMaster& T, const double* X, int g, double Xx[C
 
 // Face j, Gauss point g: unit normal and surface Jacobian.
 template <class C>
-HDGX_DEV double face_geom(const DevMaster& T, const double* X, int j, int g, double nl[C::ND]) {
+HDGX_DEV double face_geom(const double* shapmf, const int32_t* permgeom, const double* X, int j, int g, double nl[C::ND]) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NGF = C::NGF;
   double dpg[ND][ND > 1 ? ND - 1 : 1];
 #pragma unroll
@@ -114,10 +114,19 @@ HDGX_DEV double face_geom(const DevMaster& T, const double* X, int j, int g, dou
     for (int k = 0; k < ND - 1; ++k) {
       double v = 0.0;
 #pragma unroll
-      for (int a = 0; a < NPF; ++a) v += T.shapmf[g + NGF * a + NGF * NPF * (1 + k)] * X[T.permgeom[a + NPF * j] + NPV * d];
+      for (int a = 0; a < NPF; ++a) v += shapmf[g + NGF * a + NGF * NPF * (1 + k)] * X[permgeom[a + NPF * j] + NPV * d];
       dpg[d][k] = v;
     }
   return face_normal<ND>(dpg, nl);
+}
+
+template <class C>
+HDGX_DEV double vol_geom(const DevMaster& T, const double* X, int g, double Xx[C::ND][C::ND]) {
+  return vol_geom<C>(T.shapmv, X, g, Xx);
+}
+template <class C>
+HDGX_DEV double face_geom(const DevMaster& T, const double* X, int j, int g, double nl[C::ND]) {
+  return face_geom<C>(T.shapmf, T.permgeom, X, j, g, nl);
 }
 
 template <class C, class T>
@@ -359,6 +368,11 @@ struct QuadLds {
   double sSH[NPV * NC];
   int32_t sBf[NFE];
   int32_t sLoc[NPV * NFE];
+  int32_t sPerm[NPF * NFE];
+  int32_t sPermG[NPF * NFE];
+  double sShM[NGV * NPV * (ND + 1)];     // shapmv
+  double sShMF[NGF * NPF * ND];          // shapmf
+  unsigned char sAek[RO ? 4 : (S64Shape<C>::TRH * (1 + 4 * S64Shape<C>::KSMAX) + 3) / 4 * 4];
   double sShV[NGV * NPV];
   double sShF[NGF * NPF];
   double sSff[NPF * NPF * NGF];
@@ -451,6 +465,24 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     const double shv = tid < NGV * NPV ? T.shapvt[tid] : 0.0;
     const double shf = tid < NGF * NPF ? T.shapft[tid] : 0.0;
     const double sff = tid < NPF * NPF * NGF ? T.sff[tid] : 0.0;
+    // the geometry tables, node maps and (fragment-tile GK) the AE k-step table
+    constexpr int NMV = NGV * NPV * (ND + 1), NMVI = (NMV + NT - 1) / NT;
+    constexpr int NAEK = (int)sizeof(L.sAek) / 4;
+    static_assert(NGF * NPF * ND <= NT && NAEK <= NT, "staged arrays fit one workgroup pass");
+    double mv[NMVI];
+#pragma unroll
+    for (int it = 0; it < NMVI; ++it) mv[it] = tid + it * NT < NMV ? T.shapmv[tid + it * NT] : 0.0;
+    const double mf = tid < NGF * NPF * ND ? T.shapmf[tid] : 0.0;
+    const int32_t pm = tid < NPF * NFE ? T.perm[tid] : 0;
+    const int32_t pg = tid < NPF * NFE ? T.permgeom[tid] : 0;
+    const bool aek_on = TILE && !RO && !FbShape<C>::USE;
+    const int32_t ak = (aek_on && tid < NAEK) ? reinterpret_cast<const int32_t*>(T.aek[UQ ? 1 : 0])[tid] : 0;
+#pragma unroll
+    for (int it = 0; it < NMVI; ++it)
+      if (tid + it * NT < NMV) L.sShM[tid + it * NT] = mv[it];
+    if (tid < NGF * NPF * ND) L.sShMF[tid] = mf;
+    if (tid < NPF * NFE) { L.sPerm[tid] = pm; L.sPermG[tid] = pg; }
+    if (aek_on && tid < NAEK) reinterpret_cast<int32_t*>(L.sAek)[tid] = ak;
     if (tid < NPV * ncd) sX[tid] = x;
     if (tid < NPV * NC) sU[tid] = u;
     if (tid < NH) sUH[tid] = uh;
@@ -472,7 +504,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         // volume geometry: jac = det J, Xx(a, b) = jac * dxi_b/dx_a
         const int g = w;
         double Xx[ND][ND];
-        const double jac = vol_geom<C>(T, sX, g, Xx);
+        const double jac = vol_geom<C>(L.sShM, sX, g, Xx);
         sJac[g] = jac;
 #pragma unroll
         for (int a = 0; a < ND; ++a)
@@ -480,20 +512,20 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           for (int b = 0; b < ND; ++b) sXx[g * ND * ND + a * ND + b] = Xx[a][b];
         double av = 0.0;
         if (ap.av_col >= 0)
-          for (int a = 0; a < NPV; ++a) av += T.shapmv[g + NGV * a] * sX[a + NPV * ap.av_col];
+          for (int a = 0; a < NPV; ++a) av += L.sShM[g + NGV * a] * sX[a + NPV * ap.av_col];
         sAv[g] = av;
       } else if (w < W1) {
         // face geometry: unit normal and surface Jacobian
         const int t = w - W0;
         const int j = t / NGF, g = t - j * NGF;
         double nl[ND];
-        const double jac = face_geom<C>(T, sX, j, g, nl);
+        const double jac = face_geom<C>(L.sShMF, L.sPermG, sX, j, g, nl);
 #pragma unroll
         for (int i = 0; i < ND; ++i) sNl[t + NFG * i] = nl[i];
         sFJ[t] = jac;
         double av = 0.0;
         if (ap.av_col >= 0)
-          for (int a = 0; a < NPF; ++a) av += T.shapmf[g + NGF * a] * sX[T.permgeom[a + NPF * j] + NPV * ap.av_col];
+          for (int a = 0; a < NPF; ++a) av += L.sShMF[g + NGF * a] * sX[L.sPermG[a + NPF * j] + NPV * ap.av_col];
         sAvf[t] = av;
       } else if (w < W2) {
         const int i = w - W1, g = i % NGV, c = i / NGV;
@@ -506,7 +538,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         const int j = t / NGF, g = t - j * NGF;
         double v = 0.0;
 #pragma unroll
-        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sU[T.perm[a + NPF * j] + NPV * c];
+        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sU[L.sPerm[a + NPF * j] + NPV * c];
         sUf[i] = v;
       } else if (w < W4) {
         const int i = w - W3, t = i % NFG, c = i / NFG;
@@ -690,7 +722,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   // ---- 3c. Rh = -shapfg * (fbou | fhat) (faceint.m:335-339)
   double* Rh = A.Rh + e * NH;
-  for (int idx = tid; idx < NH; idx += NT) {
+  for (int idx = (tid + NT - 64) % NT; idx < NH; idx += NT) {     // from the second wavefront on (Ru takes the first)
     const int c = idx % NCU, ab = idx / NCU, j = ab / NPF, a = ab - j * NPF;
     double v = 0.0;
 #pragma unroll
@@ -822,7 +854,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       // master's AE k-step table gives each entry's column (va, k))
       using S = S64Shape<C>;
       constexpr int NV = S::KSP * S::TRH * 64;
-      const unsigned char* tab = T.aek[UQ ? 1 : 0];
+      const unsigned char* tab = L.sAek;
       double* GK = A.GK + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IG; ++it) {
