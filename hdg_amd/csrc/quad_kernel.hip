@@ -558,6 +558,48 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   __syncthreads();
 
+  // ---- 2b. M, C (volint.m:43-55), E (faceint.m:34-47): geometry only, so
+  // they run with the physics tasks (from the second wavefront on, which
+  // carries the fewest of them) and Rq needs no barrier of its own
+  auto mce = [&]() {
+    if constexpr (UQ) {
+      for (int idx = (tid + NT - 64) % NT; idx < NPV * NPV; idx += NT) {
+        double v = 0.0;
+  #pragma unroll
+        for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
+        v *= ap.fc_q;
+        sM[idx] = v;
+        if constexpr (!RO) A.M[e * NPV * NPV + idx] = v;
+      }
+      for (int idx = (tid + NT - 64) % NT; idx < NPV * NPV * ND; idx += NT) {
+        const int jjii = idx % (NPV * NPV), i = idx / (NPV * NPV);
+        double v = 0.0;
+  #pragma unroll
+        for (int j = 0; j < ND; ++j)
+  #pragma unroll
+          for (int g = 0; g < NGV; ++g) v += T.svv[jjii + NPV * NPV * (g + NGV * (j + 1))] * sXx[g * ND * ND + i * ND + j];
+        sC[idx] = v;
+        if constexpr (!RO) A.C[e * NPV * NPV * ND + idx] = v;
+      }
+      for (int idx = (tid + NT - 64) % NT; idx < NPV * NDF * ND; idx += NT) {
+        const int va = idx % NPV;
+        int rest = idx / NPV;
+        const int bj = rest % NDF, i = rest / NDF;
+        const int j = bj / NPF, b = bj - j * NPF;
+        const int la = sLoc[va + NPV * j];
+        double v = 0.0;
+        if (la >= 0)
+  #pragma unroll
+          for (int g = 0; g < NGF; ++g) {
+            const int t = j * NGF + g;
+            v += sSff[la + NPF * b + NPF * NPF * g] * (sNl[t + NFG * i] * sFJ[t]);
+          }
+        sE[idx] = v;
+        if constexpr (!RO) A.E[e * NPV * NDF * ND + idx] = v;
+      }
+    }
+  };
+
   if constexpr (RO) {
     // ---- 2. physics values only: one task per volume Gauss point, then one
     // per face point
@@ -604,6 +646,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
     }
+    mce();
   } else {
     // ---- 2. physics, one task per (Gauss point, derivative direction)
     // volume and face tasks in separate loops (their register needs do not add
@@ -701,6 +744,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
       face_out(t, k, fa, fb);
     }
+    mce();
   }
   __syncthreads();
 
@@ -728,6 +772,21 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 #pragma unroll
     for (int g = 0; g < NGF; ++g) v += T.shapfg[a + NPF * g] * sGH[j * NGF + g + NFG * c];
     Rh[idx] = RO ? v : -v;
+  }
+  if constexpr (UQ) {
+    // ---- 3h. Rq = M (SH_q/fc_q - q) + C u - E uh (volint.m:57-60, faceint.m:48-51), from the
+    // third wavefront on
+    for (int idx = (tid + NT - 128) % NT; idx < NPV * NCU * ND; idx += NT) {
+      const int a = idx % NPV;
+      int rest = idx / NPV;
+      const int c = rest % NCU, i = rest / NCU;
+      const int qc = NCU + c + NCU * i;
+      double v = 0.0, w = 0.0, z = 0.0;
+      for (int b = 0; b < NPV; ++b) v += sM[a + NPV * b] * (sSH[b + NPV * qc] / ap.fc_q - sU[b + NPV * qc]);
+      for (int b = 0; b < NPV; ++b) w += sC[a + NPV * b + NPV * NPV * i] * sU[b + NPV * c];
+      for (int b = 0; b < NDF; ++b) z += sE[a + NPV * b + NPV * NDF * i] * sUH[c + NCU * b];
+      A.Rq[e * NPV * NCU * ND + idx] = v + w - z;
+    }
   }
   if constexpr (!RO) {
     // ---- 3a. BD (npv,ncu,npv,nc) = volume + face blocks (volint.m:262,
@@ -908,56 +967,6 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         rest /= NDF;
         H[idx] = h_entry(c, aj, rest % NCU, rest / NCU);
       }
-    }
-  }
-  if constexpr (UQ) {
-    // ---- 3g. M, C (volint.m:43-55), E (faceint.m:34-47)
-    for (int idx = tid; idx < NPV * NPV; idx += NT) {
-      double v = 0.0;
-#pragma unroll
-      for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
-      v *= ap.fc_q;
-      sM[idx] = v;
-      if constexpr (!RO) A.M[e * NPV * NPV + idx] = v;
-    }
-    for (int idx = tid; idx < NPV * NPV * ND; idx += NT) {
-      const int jjii = idx % (NPV * NPV), i = idx / (NPV * NPV);
-      double v = 0.0;
-#pragma unroll
-      for (int j = 0; j < ND; ++j)
-#pragma unroll
-        for (int g = 0; g < NGV; ++g) v += T.svv[jjii + NPV * NPV * (g + NGV * (j + 1))] * sXx[g * ND * ND + i * ND + j];
-      sC[idx] = v;
-      if constexpr (!RO) A.C[e * NPV * NPV * ND + idx] = v;
-    }
-    for (int idx = tid; idx < NPV * NDF * ND; idx += NT) {
-      const int va = idx % NPV;
-      int rest = idx / NPV;
-      const int bj = rest % NDF, i = rest / NDF;
-      const int j = bj / NPF, b = bj - j * NPF;
-      const int la = sLoc[va + NPV * j];
-      double v = 0.0;
-      if (la >= 0)
-#pragma unroll
-        for (int g = 0; g < NGF; ++g) {
-          const int t = j * NGF + g;
-          v += sSff[la + NPF * b + NPF * NPF * g] * (sNl[t + NFG * i] * sFJ[t]);
-        }
-      sE[idx] = v;
-      if constexpr (!RO) A.E[e * NPV * NDF * ND + idx] = v;
-    }
-    __syncthreads();
-    // ---- 3h. Rq = M (SH_q/fc_q - q) + C u - E uh (volint.m:57-60, faceint.m:48-51)
-    for (int idx = tid; idx < NPV * NCU * ND; idx += NT) {
-      const int a = idx % NPV;
-      int rest = idx / NPV;
-      const int c = rest % NCU, i = rest / NCU;
-      const int qc = NCU + c + NCU * i;
-      double v = 0.0, w = 0.0, z = 0.0;
-      for (int b = 0; b < NPV; ++b) v += sM[a + NPV * b] * (sSH[b + NPV * qc] / ap.fc_q - sU[b + NPV * qc]);
-      for (int b = 0; b < NPV; ++b) w += sC[a + NPV * b + NPV * NPV * i] * sU[b + NPV * c];
-      for (int b = 0; b < NDF; ++b) z += sE[a + NPV * b + NPV * NDF * i] * sUH[c + NCU * b];
-      A.Rq[e * NPV * NCU * ND + idx] = v + w - z;
     }
   }
 }
