@@ -338,9 +338,15 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
           const double pv = lane_bcast(pr[t], p);
           const double ip = 1.0 / pv;
           z[t] = (lane == p) ? 1.0 : 0.0;
+          // the pivot row's later panel columns and its multipliers of the
+          // earlier pivots (its z(t) is 1, later z 0; a row's earlier columns
+          // are not read again)
           double ppr[4], pz[4];
 #pragma unroll
-          for (int t2 = 0; t2 < 4; ++t2) { ppr[t2] = lane_bcast(pr[t2], p); pz[t2] = lane_bcast(z[t2], p); }
+          for (int t2 = 0; t2 < 4; ++t2) {
+            ppr[t2] = t2 > t ? lane_bcast(pr[t2], p) : 0.0;
+            pz[t2] = t2 < t ? lane_bcast(z[t2], p) : (t2 == t ? 1.0 : 0.0);
+          }
           if (lane == p) {
 #pragma unroll
             for (int t2 = 0; t2 < 4; ++t2) { pr[t2] *= ip; z[t2] *= ip; }
@@ -348,7 +354,10 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
           } else {
             const double m = pr[t] * ip;
 #pragma unroll
-            for (int t2 = 0; t2 < 4; ++t2) { pr[t2] -= m * ppr[t2]; z[t2] -= m * pz[t2]; }
+            for (int t2 = 0; t2 < 4; ++t2) {
+              if (t2 > t) pr[t2] -= m * ppr[t2];
+              z[t2] -= m * pz[t2];
+            }
           }
           if (lane == 0) sStep[p] = k0 + t;
         }
