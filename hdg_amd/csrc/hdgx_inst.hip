@@ -16,12 +16,12 @@ template <class C>
 hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap, bool tile) {
   if constexpr (WsLayout<C>::TILE_OK) {
     if (tile) {
-      hipLaunchKernelGGL((quad_kernel<C, true>), dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+      hipLaunchKernelGGL((quad_kernel<C, true>), dim3((unsigned)a.ne), dim3(QuadThreads<C>::N), 0, s, a, t, ap);
       return hipGetLastError();
     }
   }
   if (tile) return hipErrorInvalidValue;
-  hipLaunchKernelGGL((quad_kernel<C, false>), dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+  hipLaunchKernelGGL((quad_kernel<C, false>), dim3((unsigned)a.ne), dim3(QuadThreads<C>::N), 0, s, a, t, ap);
   return hipGetLastError();
 }
 
@@ -32,7 +32,7 @@ size_t ws_words() {
 
 template <class C>
 hipError_t resid_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap) {
-  hipLaunchKernelGGL(resid_kernel<C>, dim3((unsigned)a.ne), dim3(QUAD_THREADS), 0, s, a, t, ap);
+  hipLaunchKernelGGL(resid_kernel<C>, dim3((unsigned)a.ne), dim3(QuadThreads<C>::N), 0, s, a, t, ap);
   return hipGetLastError();
 }
 
@@ -89,7 +89,7 @@ hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, 
 template <class C>
 hipError_t seg_launch(hipStream_t s, const SegTable& t, const DevMaster& m, const int32_t* perm) {
   if (t.total <= 0) return hipSuccess;
-  hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QUAD_THREADS), 0, s, t, m);
+  hipLaunchKernelGGL(quad_seg_kernel<C>, dim3((unsigned)t.total), dim3(QuadThreads<C>::N), 0, s, t, m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if constexpr (InvShape<C>::USE) {

@@ -32,6 +32,29 @@ namespace hdgx {
 
 constexpr int QUAD_THREADS = 256;
 
+// Threads of one element's workgroup: 64 where one wavefront stages every
+// input array in one pass and takes each task phase (pointwise inputs,
+// physics) in at most two rounds, else 256.  The smallest orders then keep
+// many elements in flight per CU instead of one 256-thread workgroup that
+// idles (2D Euler p = 1: 0.57 -> 0.36 ms for 80 000 elements); at p = 2 and 3
+// fewer threads a workgroup measured no faster (one wavefront) or slower (two).
+template <class C>
+struct QuadThreads {
+  static constexpr int cdiv(int a, int b) { return (a + b - 1) / b; }
+  static constexpr int mx(int a, int b) { return a > b ? a : b; }
+  static constexpr int need() {
+    constexpr int NFG = C::NFE * C::NGF;
+    int n = mx(C::NPV * (C::ND + 2), C::NPV * C::NC);
+    n = mx(n, mx(C::NH, C::NPV * C::NFE));
+    n = mx(n, mx(C::NGV * C::NPV, C::NPF * C::NPF * C::NGF));
+    n = mx(n, C::NGF * C::NPF * C::ND);
+    const int p1 = C::NGV + NFG + C::NGV * C::NC + NFG * C::NC + NFG * C::NCU + C::NGV * C::NCU;
+    const int p2 = C::NGV * C::NC + NFG * (C::NC + C::NCU);
+    return mx(n, mx(cdiv(p1, 2), cdiv(p2, 2)));
+  }
+  static constexpr int N = need() <= 64 ? 64 : 256;
+};
+
 typedef Dual<1> D1;
 
 // Loads through the constant address space: with a wavefront-uniform index
@@ -400,16 +423,17 @@ struct QuadLds {
   double sE[UQ ? NPV * NDF * ND : 1];
 };
 
-// Element e of the batch A (a workgroup of QUAD_THREADS threads), LDS in L.
+// Element e of the batch A (a workgroup of NT_ threads), LDS in L.
 // RO: the residual only (hdg_residual.m:1-57: Ru, Rq and Rh with that
 // function's sign, +shapfg (fbou | fhat), hdg_residual.m:286-290), no
 // Jacobians and no element matrices.
-template <class C, bool RO = false, bool TILE = false>
+template <class C, bool RO = false, bool TILE = false, int NT_ = QUAD_THREADS>
 HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap, const int64_t e,
                         QuadLds<C, RO>& L) {
   constexpr int ND = C::ND, NPV = C::NPV, NPF = C::NPF, NFE = C::NFE, NDF = C::NDF;
   constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NU = C::NU, NH = C::NH;
-  constexpr int NR = C::NR, NFG = NFE * NGF, NT = QUAD_THREADS;
+  constexpr int NR = C::NR, NFG = NFE * NGF, NT = NT_;
+  static_assert(NT % 64 == 0 && NT <= 256, "whole wavefronts");
   constexpr int NCC = NCU * NC;          // BD columns per volume node pair: (c, k)
   constexpr int NDIRF = NC + NCU;        // face derivative directions: UDG components, then UH
   constexpr bool UQ = C::UQ;
@@ -563,7 +587,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   // carries the fewest of them) and Rq needs no barrier of its own
   auto mce = [&]() {
     if constexpr (UQ) {
-      for (int idx = (tid + NT - 64) % NT; idx < NPV * NPV; idx += NT) {
+      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NPV; idx += NT) {
         double v = 0.0;
   #pragma unroll
         for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
@@ -571,7 +595,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         sM[idx] = v;
         if constexpr (!RO) A.M[e * NPV * NPV + idx] = v;
       }
-      for (int idx = (tid + NT - 64) % NT; idx < NPV * NPV * ND; idx += NT) {
+      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NPV * ND; idx += NT) {
         const int jjii = idx % (NPV * NPV), i = idx / (NPV * NPV);
         double v = 0.0;
   #pragma unroll
@@ -581,7 +605,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         sC[idx] = v;
         if constexpr (!RO) A.C[e * NPV * NPV * ND + idx] = v;
       }
-      for (int idx = (tid + NT - 64) % NT; idx < NPV * NDF * ND; idx += NT) {
+      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NDF * ND; idx += NT) {
         const int va = idx % NPV;
         int rest = idx / NPV;
         const int bj = rest % NDF, i = rest / NDF;
@@ -766,7 +790,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   // ---- 3c. Rh = -shapfg * (fbou | fhat) (faceint.m:335-339)
   double* Rh = A.Rh + e * NH;
-  for (int idx = (tid + NT - 64) % NT; idx < NH; idx += NT) {     // from the second wavefront on (Ru takes the first)
+  for (int idx = (tid + NT - 64 % NT) % NT; idx < NH; idx += NT) {     // from the second wavefront on (Ru takes the first)
     const int c = idx % NCU, ab = idx / NCU, j = ab / NPF, a = ab - j * NPF;
     double v = 0.0;
 #pragma unroll
@@ -776,7 +800,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   if constexpr (UQ) {
     // ---- 3h. Rq = M (SH_q/fc_q - q) + C u - E uh (volint.m:57-60, faceint.m:48-51), from the
     // third wavefront on
-    for (int idx = (tid + NT - 128) % NT; idx < NPV * NCU * ND; idx += NT) {
+    for (int idx = (tid + NT - 128 % NT) % NT; idx < NPV * NCU * ND; idx += NT) {
       const int a = idx % NPV;
       int rest = idx / NPV;
       const int c = rest % NCU, i = rest / NCU;
@@ -808,8 +832,10 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
       for (int it = 0; it < n; ++it) k[it] = tid + it * NT < nv ? key[tid + it * NT] : -1;
     };
-    int kf[IF], kg[IG], kh[IH];
-    if constexpr (TILE) load_keys(FghKeyTab<C, FBK>::tab.f, KT_::NF, kf, IF);
+    // in registers when they fit (at most 32 a thread), else loaded in the loops
+    constexpr bool KPRE = IF + IG + IH <= 32;
+    int kf[KPRE ? IF : 1], kg[KPRE ? IG : 1], kh[KPRE ? IH : 1];
+    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK>::tab.f, KT_::NF, kf, IF);
     using BS = BdShape<C>;
     constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
     double* BD = A.BD + e * (int64_t)NU * C::NCOL;
@@ -835,7 +861,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
     }
-    if constexpr (TILE) load_keys(FghKeyTab<C, FBK>::tab.gk, KT_::NG, kg, IG);
+    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK>::tab.gk, KT_::NG, kg, IG);
     // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327); F(va, c, k, bj) is u row
     // va + NPV c, trace column k + NCU bj
     auto f_entry = [&](int va, int c, int k, int bj) {
@@ -858,10 +884,11 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       // face blocks: F_f(q, r) = F(u row of q on face f, trace column f NB + r)
       constexpr int NV = (int)FbShape<C>::BLK;
       double* F = A.F + e * (int64_t)NV;
-      load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
   #pragma unroll
       for (int it = 0; it < IF; ++it) {
-        const int idx = tid + it * NT, kv = kf[it];
+        const int idx = tid + it * NT;
+        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.f[idx] : -1);
         if (idx < NV) F[idx] = keyed(kv & 255, sFHh + (kv >> 8));
       }
     } else if constexpr (TILE) {
@@ -869,10 +896,11 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       using S = S64Shape<C>;
       constexpr int NV = (NU / 4) * S::TRH * 64;
       double* F = A.F + e * (int64_t)NV;
-      load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
   #pragma unroll
       for (int it = 0; it < IF; ++it) {
-        const int idx = tid + it * NT, kv = kf[it];
+        const int idx = tid + it * NT;
+        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.f[idx] : -1);
         if (idx >= NV) break;
         const int la = kv >= 0 ? sLoc[kv & 255] : -1;
         F[idx] = la >= 0 ? keyed(la + NPF * ((kv >> 8) & 15), sFHh + (kv >> 12)) : 0.0;
@@ -905,7 +933,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       double* GK = A.GK + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IG; ++it) {
-        const int idx = tid + it * NT, kv = kg[it];
+        const int idx = tid + it * NT;
+        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.gk[idx] : -1);
         if (idx < NV) GK[idx] = keyed(kv & 255, sGHu + (kv >> 8));
       }
     } else if constexpr (TILE) {
@@ -917,7 +946,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       double* GK = A.GK + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IG; ++it) {
-        const int idx = tid + it * NT, kv = kg[it];
+        const int idx = tid + it * NT;
+        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.gk[idx] : -1);
         if (idx >= NV) break;
         const int kcol = kv >= 0 ? tab[kv & 1023] : 255;
         double v = 0.0;
@@ -955,7 +985,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       double* H = A.H + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IH; ++it) {
-        const int idx = tid + it * NT, kv = kh[it];
+        const int idx = tid + it * NT;
+        const int kv = KPRE ? kh[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.h[idx] : -1);
         if (idx < NV) H[idx] = kv >= 0 ? keyed(kv & 255, sGHh + (kv >> 8)) : 0.0;
       }
     } else {
@@ -972,16 +1003,16 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 }
 
 template <class C, bool TILE = false>
-__global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+__global__ __launch_bounds__(QuadThreads<C>::N) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C> lds;
-  quad_body<C, false, TILE>(A, T, ap, blockIdx.x, lds);
+  quad_body<C, false, TILE, QuadThreads<C>::N>(A, T, ap, blockIdx.x, lds);
 }
 
 // residual only: Ru, Rh (hdg_residual.m's sign), Rq
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) __attribute__((amdgpu_waves_per_eu(5))) void resid_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+__global__ __launch_bounds__(QuadThreads<C>::N) __attribute__((amdgpu_waves_per_eu(5))) void resid_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C, true> lds;
-  quad_body<C, true>(A, T, ap, blockIdx.x, lds);
+  quad_body<C, true, false, QuadThreads<C>::N>(A, T, ap, blockIdx.x, lds);
 }
 
 }  // namespace hdgx

@@ -34,7 +34,7 @@ HDGX_DEV int find_seg(const ElemSeg* segs, int nseg, int64_t g) {
 }
 
 template <class C>
-__global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevMaster T) {
+__global__ __launch_bounds__(QuadThreads<C>::N) void quad_seg_kernel(SegTable S, DevMaster T) {
   constexpr int NPV = C::NPV, NFE = C::NFE, NC = C::NC, NU = C::NU, NH = C::NH, NCOL = C::NCOL;
   constexpr int ND = C::ND, NDF = C::NDF, NCU = C::NCU;
   const int64_t w = blockIdx.x;
@@ -70,7 +70,7 @@ __global__ __launch_bounds__(QUAD_THREADS) void quad_seg_kernel(SegTable S, DevM
   }
   A.status = nullptr;
   __shared__ QuadLds<C> lds;
-  quad_body<C, false, TILE>(A, T, ap, 0, lds);
+  quad_body<C, false, TILE, QuadThreads<C>::N>(A, T, ap, 0, lds);
 }
 
 // the elimination arguments of batch element g (workspace slot w) as a
