@@ -398,13 +398,18 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
               if (ct >= ct0) sRows[t * RSTR + 16 * ct + c] = tmp[ct];
         }
         __syncthreads();
-        // S += Z * S[pivot rows, :]
+        // S += Z * S[pivot rows, :], the panel's own column tile first: the
+        // next panel reads it, and the other tiles' products run on the MFMA
+        // pipe behind that panel's pivot searches
+        double za[RT];
 #pragma unroll
-        for (int rt = 0; rt < RT; ++rt) {
-          const double a = sZ[g * RP + 16 * rt + c];
+        for (int rt = 0; rt < RT; ++rt) za[rt] = sZ[g * RP + 16 * rt + c];
 #pragma unroll
-          for (int ct = 0; ct < CT; ++ct)
-            if (ct >= ct0) acc[rt][ct] = mfma64(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
+        for (int ct = 0; ct < CT; ++ct) {
+          if (ct < ct0) continue;
+          const double b = sRows[g * RSTR + 16 * ct + c];
+#pragma unroll
+          for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(za[rt], b, acc[rt][ct]);
         }
         __syncthreads();
       }

@@ -190,12 +190,17 @@ HDGX_DEV void inv64_body(const SchurArgs& A, const int64_t e, const int32_t* per
         }
       }
       __syncthreads();
-      // S += Z * S[pivot rows, :]
+      // S += Z * S[pivot rows, :], the panel's own column tile first (the
+      // next panel reads it; the rest runs behind its pivot searches)
+      double za[RT];
 #pragma unroll
-      for (int rt = 0; rt < RT; ++rt) {
-        const double a = sZ[g * RP + 16 * rt + c];
+      for (int rt = 0; rt < RT; ++rt) za[rt] = sZ[g * RP + 16 * rt + c];
 #pragma unroll
-        for (int ct = 0; ct < RT; ++ct) acc[rt][ct] = mfma64(a, sRows[g * RSTR + 16 * ct + c], acc[rt][ct]);
+      for (int k = 0; k < RT; ++k) {
+        const int ct = (ct0 + k) % RT;
+        const double b = sRows[g * RSTR + 16 * ct + c];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[rt][ct] = mfma64(za[rt], b, acc[rt][ct]);
       }
       __syncthreads();
     }
