@@ -519,12 +519,43 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   }
   __syncthreads();
 
-  // ---- 1. pointwise inputs
+  // ---- 1. pointwise inputs: the state, trace and SH at every Gauss point
+  // (one task per point and component), then the volume and face geometry on
+  // the last threads, which the interpolation's last round leaves idle
   {
-    constexpr int W0 = NGV, W1 = W0 + NFG, W2 = W1 + NGV * NC, W3 = W2 + NFG * NC, W4 = W3 + NFG * NCU;
-    const int W5 = W4 + (tdep ? NGV * NCU : 0);
-    for (int w = tid; w < W5; w += NT) {
-      if (w < W0) {
+    constexpr int W1 = NGV * NC, W2 = W1 + NFG * NC, W3 = W2 + NFG * NCU;
+    const int W4 = W3 + (tdep ? NGV * NCU : 0);
+    for (int w = tid; w < W4; w += NT) {
+      if (w < W1) {
+        const int g = w % NGV, c = w / NGV;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sU[a + NPV * c];
+        sUg[w] = v;
+      } else if (w < W2) {
+        const int i = w - W1, t = i % NFG, c = i / NFG;
+        const int j = t / NGF, g = t - j * NGF;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sU[L.sPerm[a + NPF * j] + NPV * c];
+        sUf[i] = v;
+      } else if (w < W3) {
+        const int i = w - W2, t = i % NFG, c = i / NFG;
+        const int j = t / NGF, g = t - j * NGF;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sUH[c + NCU * (a + NPF * j)];
+        sUHf[i] = v;
+      } else {
+        const int i = w - W3, g = i % NGV, c = i / NGV;
+        double v = 0.0;
+#pragma unroll
+        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sSH[a + NPV * c];
+        sSg[i] = v;
+      }
+    }
+    for (int w = NT - 1 - tid; w < NGV + NFG; w += NT) {
+      if (w < NGV) {
         // volume geometry: jac = det J, Xx(a, b) = jac * dxi_b/dx_a
         const int g = w;
         double Xx[ND][ND];
@@ -538,9 +569,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         if (ap.av_col >= 0)
           for (int a = 0; a < NPV; ++a) av += L.sShM[g + NGV * a] * sX[a + NPV * ap.av_col];
         sAv[g] = av;
-      } else if (w < W1) {
+      } else {
         // face geometry: unit normal and surface Jacobian
-        const int t = w - W0;
+        const int t = w - NGV;
         const int j = t / NGF, g = t - j * NGF;
         double nl[ND];
         const double jac = face_geom<C>(L.sShMF, L.sPermG, sX, j, g, nl);
@@ -551,32 +582,6 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         if (ap.av_col >= 0)
           for (int a = 0; a < NPF; ++a) av += L.sShMF[g + NGF * a] * sX[L.sPermG[a + NPF * j] + NPV * ap.av_col];
         sAvf[t] = av;
-      } else if (w < W2) {
-        const int i = w - W1, g = i % NGV, c = i / NGV;
-        double v = 0.0;
-#pragma unroll
-        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sU[a + NPV * c];
-        sUg[i] = v;
-      } else if (w < W3) {
-        const int i = w - W2, t = i % NFG, c = i / NFG;
-        const int j = t / NGF, g = t - j * NGF;
-        double v = 0.0;
-#pragma unroll
-        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sU[L.sPerm[a + NPF * j] + NPV * c];
-        sUf[i] = v;
-      } else if (w < W4) {
-        const int i = w - W3, t = i % NFG, c = i / NFG;
-        const int j = t / NGF, g = t - j * NGF;
-        double v = 0.0;
-#pragma unroll
-        for (int a = 0; a < NPF; ++a) v += sShF[g + NGF * a] * sUH[c + NCU * (a + NPF * j)];
-        sUHf[i] = v;
-      } else {
-        const int i = w - W4, g = i % NGV, c = i / NGV;
-        double v = 0.0;
-#pragma unroll
-        for (int a = 0; a < NPV; ++a) v += sShV[g + NGV * a] * sSH[a + NPV * c];
-        sSg[i] = v;
       }
     }
   }
