@@ -69,8 +69,8 @@ struct S64Shape {
   static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
 };
 
-// Face-block form of the u-only four-row-tile systems whose faces pad to whole
-// 16-row tiles (Euler tetrahedra p = 2: 30 trace rows per face, two tiles):
+// Face-block form of the u-only four-row-tile systems (Euler tetrahedra p = 2:
+// 30 trace rows per face, two tiles; 2D Euler p = 4: 20 rows, two tiles):
 // F, GK and H are nonzero only where a trace row or column and a volume node
 // share a face (faceint.m:316-359; H block diagonal per face), so each is kept
 // as NFE blocks of NB x NB, NB = NCU * NPF.  Face-local indices: trace r =
@@ -83,7 +83,9 @@ struct FbShape {
   static constexpr int NB = C::NCU * C::NPF;
   static constexpr int NBT = (NB + 15) / 16;     // 16-row tiles per face
   static constexpr int KSF = (NB + 3) / 4;       // face-local k-steps
-  static constexpr bool USE = S::USE && S::RT == 4 && !C::UQ && NBT * C::NFE <= S::TRH;
+  // where the blocks hold fewer words than the dense arrays (Euler tetrahedra
+  // p = 2: 3 600 of 6 000; 2D Euler p = 4: 1 200 of 3 600)
+  static constexpr bool USE = S::USE && S::RT == 4 && !C::UQ && C::NFE * NB * NB < C::NH * C::NU;
   static constexpr size_t BLK = (size_t)C::NFE * NB * NB;   // words of F, GK or H
 };
 
