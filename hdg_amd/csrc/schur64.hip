@@ -579,16 +579,31 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         }
       __syncthreads();
     }
-    // DUDG_DUH columns of this tile, coalesced over rows
+    // DUDG_DUH columns of this tile, coalesced over rows (two rows a lane
+    // where the rows and the array allow 16-byte stores)
     if (LDO > 0) {
       double* Xe = A.dudg_duh + e * (int64_t)LDO * NH;
-      for (int cc = 0; cc < 16; ++cc) {
-        const int jj = 16 * tbg + cc;
-        if (jj >= NH) break;
-        for (int r = lane; r < LDO; r += 64) {
-          const double v = sX[r + NCOLP * cc];
-          if (!isfinite(v)) st |= 4;
-          Xe[r + (int64_t)LDO * s64_t<C>(jj)] = v;
+      if (LDO % 2 == 0 && (reinterpret_cast<uintptr_t>(A.dudg_duh) & 15) == 0) {
+        for (int cc = 0; cc < 16; ++cc) {
+          const int jj = 16 * tbg + cc;
+          if (jj >= NH) break;
+          for (int r2 = lane; 2 * r2 < LDO; r2 += 64) {
+            double2 v;
+            v.x = sX[2 * r2 + NCOLP * cc];
+            v.y = sX[2 * r2 + 1 + NCOLP * cc];
+            if (!isfinite(v.x) || !isfinite(v.y)) st |= 4;
+            *reinterpret_cast<double2*>(Xe + 2 * r2 + (int64_t)LDO * s64_t<C>(jj)) = v;
+          }
+        }
+      } else {
+        for (int cc = 0; cc < 16; ++cc) {
+          const int jj = 16 * tbg + cc;
+          if (jj >= NH) break;
+          for (int r = lane; r < LDO; r += 64) {
+            const double v = sX[r + NCOLP * cc];
+            if (!isfinite(v)) st |= 4;
+            Xe[r + (int64_t)LDO * s64_t<C>(jj)] = v;
+          }
         }
       }
     }
@@ -639,15 +654,30 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         if (row < NH) sX[row + NHP * c] = ae[rta][i];
       }
     __syncthreads();
-    // whole AE columns: NH contiguous rows each
+    // whole AE columns: NH contiguous rows each (NH is a multiple of 4), two
+    // rows a lane where the array allows 16-byte stores
     double* AEe = A.ae + e * (int64_t)NH * NH;
-    for (int idx = lane; idx < 16 * NH; idx += 64) {
-      const int cc = idx / NH, row = idx - cc * NH;
-      const int jj = 16 * tbg + cc;
-      if (jj < NH) {
-        const double v = sX[row + NHP * cc];
-        if (!isfinite(v)) st |= 4;
-        AEe[row + NH * s64_t<C>(jj)] = v;
+    if ((reinterpret_cast<uintptr_t>(A.ae) & 15) == 0) {
+      for (int idx = lane; idx < 8 * NH; idx += 64) {
+        const int cc = idx / (NH / 2), row = 2 * (idx - cc * (NH / 2));
+        const int jj = 16 * tbg + cc;
+        if (jj < NH) {
+          double2 v;
+          v.x = sX[row + NHP * cc];
+          v.y = sX[row + 1 + NHP * cc];
+          if (!isfinite(v.x) || !isfinite(v.y)) st |= 4;
+          *reinterpret_cast<double2*>(AEe + row + NH * s64_t<C>(jj)) = v;
+        }
+      }
+    } else {
+      for (int idx = lane; idx < 16 * NH; idx += 64) {
+        const int cc = idx / NH, row = idx - cc * NH;
+        const int jj = 16 * tbg + cc;
+        if (jj < NH) {
+          const double v = sX[row + NHP * cc];
+          if (!isfinite(v)) st |= 4;
+          AEe[row + NH * s64_t<C>(jj)] = v;
+        }
       }
     }
     __syncthreads();
