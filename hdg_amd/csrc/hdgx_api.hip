@@ -39,9 +39,19 @@ int64_t ws_chunk(int64_t n, size_t bytes_per_element, size_t cap) {
   return (n + nch - 1) / nch;
 }
 
+// FNV-1a over 8-byte words (then the tail bytes), with a shift mixing each
+// step: the master-table cache key, computed on every call
 uint64_t fnv(uint64_t h, const void* p, size_t n) {
   const unsigned char* c = static_cast<const unsigned char*>(p);
-  for (size_t i = 0; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, c + i, 8);
+    h ^= w;
+    h *= 1099511628211ull;
+    h ^= h >> 29;
+  }
+  for (; i < n; ++i) { h ^= c[i]; h *= 1099511628211ull; }
   return h;
 }
 
@@ -116,7 +126,9 @@ std::vector<double> bd_table(const hdgx_master* m, const std::vector<int32_t>& l
   return t;
 }
 
-int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
+// Checks a master's sizes and table pointers and hashes its content (the
+// cache key of its device image).
+int master_hash(hdgx_ctx* ctx, const hdgx_master* m, uint64_t* out) {
   if (!m || !m->shapvt || !m->shapvg || !m->shapvgdotshapvl || !m->shapft || !m->shapfg || !m->shapfgdotshapfc ||
       !m->shapmv || !m->shapmf || !m->perm || !m->permgeom)
     return fail(ctx, HDGX_ERR_ARG, "master: null table");
@@ -126,7 +138,7 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   const size_t n_vt = (size_t)ngv * npv * (nd + 1), n_vg = (size_t)npv * ngv * (nd + 1);
   const size_t n_vv = (size_t)npv * npv * ngv * (nd + 1), n_ft = (size_t)ngf * npf * nd;
   const size_t n_fg = (size_t)npf * ngf * nd, n_ff = (size_t)npf * npf * ngf * nd;
-  const size_t n_mv = n_vt, n_mf = n_ft, n_perm = (size_t)npf * nfe, n_loc = (size_t)npv * nfe;
+  const size_t n_mv = n_vt, n_mf = n_ft, n_perm = (size_t)npf * nfe;
   uint64_t h = 1469598103934665603ull;
   int dims[7] = {nd, m->porder, npv, npf, nfe, ngv, ngf};
   h = fnv(h, dims, sizeof(dims));
@@ -134,7 +146,18 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   h = fnv(h, m->shapft, n_ft * 8); h = fnv(h, m->shapfg, n_fg * 8); h = fnv(h, m->shapfgdotshapfc, n_ff * 8);
   h = fnv(h, m->shapmv, n_mv * 8); h = fnv(h, m->shapmf, n_mf * 8);
   h = fnv(h, m->perm, n_perm * 4); h = fnv(h, m->permgeom, n_perm * 4);
-  img->hash = h;
+  *out = h;
+  return HDGX_OK;
+}
+
+int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
+  int rc = master_hash(ctx, m, &img->hash);
+  if (rc) return rc;
+  const int nd = m->nd, npv = m->npv, npf = m->npf, nfe = m->nfe, ngv = m->ngv, ngf = m->ngf;
+  const size_t n_vt = (size_t)ngv * npv * (nd + 1), n_vg = (size_t)npv * ngv * (nd + 1);
+  const size_t n_vv = (size_t)npv * npv * ngv * (nd + 1), n_ft = (size_t)ngf * npf * nd;
+  const size_t n_fg = (size_t)npf * ngf * nd, n_ff = (size_t)npf * npf * ngf * nd;
+  const size_t n_mv = n_vt, n_mf = n_ft, n_perm = (size_t)npf * nfe, n_loc = (size_t)npv * nfe;
   std::vector<int32_t> loc(n_loc, -1);
   for (int j = 0; j < nfe; ++j)
     for (int a = 0; a < npf; ++a) {
@@ -185,11 +208,14 @@ void bind_master(const hdgx_master* m, const MasterImage& img, char* base, Maste
 
 // Upload (once per distinct table content and context) the master tables.
 int get_master(hdgx_ctx* ctx, const hdgx_master* m, MasterBuf** out) {
-  MasterImage img;
-  int rc = pack_master(ctx, m, &img);
+  uint64_t hash = 0;
+  int rc = master_hash(ctx, m, &hash);
   if (rc) return rc;
-  auto it = ctx->masters.find(img.hash);
+  auto it = ctx->masters.find(hash);
   if (it != ctx->masters.end()) { *out = &it->second; return HDGX_OK; }
+  MasterImage img;
+  rc = pack_master(ctx, m, &img);
+  if (rc) return rc;
   MasterBuf mb;
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   HIP_TRY(ctx, hipMalloc(&mb.dev, img.host.size()));
@@ -897,10 +923,16 @@ int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_
   if (d.state != 1) return fail(ctx, HDGX_ERR_DEVICE, "elemental_noalloc: hipGetSymbolAddress of the scratch array failed");
   HIP_TRY(ctx, hipSetDevice(ctx->device));
   MasterImage img;
-  int rc = pack_master(ctx, master, &img);
+  // the image is built only when the table area holds another master's
+  uint64_t hash = 0;
+  int rc = master_hash(ctx, master, &hash);
   if (rc) return rc;
-  if (img.host.size() > NOALLOC_TABLE_BYTES)
-    return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: master tables exceed the 1 MiB table area");
+  const bool fresh = d.table_hash != hash;
+  if (fresh) {
+    if ((rc = pack_master(ctx, master, &img))) return rc;
+    if (img.host.size() > NOALLOC_TABLE_BYTES)
+      return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: master tables exceed the 1 MiB table area");
+  }
   DevApp ap;
   if ((rc = make_app(ctx, master, app, &ap))) return rc;
   if (app->tdep && !sh) return fail(ctx, HDGX_ERR_ARG, "tdep needs sh");
@@ -909,7 +941,7 @@ int hdgx_elemental_noalloc(hdgx_ctx* ctx, const hdgx_master* master, const hdgx_
   const size_t ws_bytes = NOALLOC_BYTES - NOALLOC_TABLE_BYTES;
   if (per > ws_bytes) return fail(ctx, HDGX_ERR_ARG, "elemental_noalloc: one element's matrices exceed the scratch");
   if (ne == 0) return HDGX_OK;
-  if (d.table_hash != img.hash) {
+  if (fresh) {
     // stream-ordered after earlier calls on this stream; the host image is a
     // local, so the copy completes before the call goes on
     HIP_TRY(ctx, hipMemcpyAsync(d.ptr, img.host.data(), img.host.size(), hipMemcpyHostToDevice, ctx->stream));
