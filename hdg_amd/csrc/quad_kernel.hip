@@ -588,11 +588,15 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   __syncthreads();
 
   // ---- 2b. M, C (volint.m:43-55), E (faceint.m:34-47): geometry only, so
-  // they run with the physics tasks (from the second wavefront on, which
-  // carries the fewest of them) and Rq needs no barrier of its own
+  // they run with the physics tasks (on the wavefronts that carry the fewest
+  // of them) and Rq needs no barrier of its own
   auto mce = [&]() {
     if constexpr (UQ) {
-      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NPV; idx += NT) {
+      // from the wavefront after the one with the last physics chunk
+      constexpr int NCH = (NGV * NC + 63) / 64 + (NFG * NC + 63) / 64 + (NFG * NCU + 63) / 64;
+      constexpr int MOFF = 64 * (NCH % (NT / 64));
+      const int r0 = (tid + NT - MOFF) % NT;
+      for (int idx = r0; idx < NPV * NPV; idx += NT) {
         double v = 0.0;
   #pragma unroll
         for (int g = 0; g < NGV; ++g) v += T.svv[idx + NPV * NPV * g] * sJac[g];
@@ -600,7 +604,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         sM[idx] = v;
         if constexpr (!RO) A.M[e * NPV * NPV + idx] = v;
       }
-      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NPV * ND; idx += NT) {
+      for (int idx = r0; idx < NPV * NPV * ND; idx += NT) {
         const int jjii = idx % (NPV * NPV), i = idx / (NPV * NPV);
         double v = 0.0;
   #pragma unroll
@@ -610,7 +614,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         sC[idx] = v;
         if constexpr (!RO) A.C[e * NPV * NPV * ND + idx] = v;
       }
-      for (int idx = (tid + NT - 64 % NT) % NT; idx < NPV * NDF * ND; idx += NT) {
+      for (int idx = r0; idx < NPV * NDF * ND; idx += NT) {
         const int va = idx % NPV;
         int rest = idx / NPV;
         const int bj = rest % NDF, i = rest / NDF;
@@ -680,8 +684,17 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     // ---- 2. physics, one task per (Gauss point, derivative direction)
     // volume and face tasks in separate loops (their register needs do not add
     // up); task w is thread w mod NT's as in one loop over both
-    for (int w = tid; w < NGV * NC; w += NT) {
-      {
+    // the tasks of each kind in chunks of 64, dealt to the wavefronts in turn
+    // (volume chunks first, then face-UDG, then face-UH): a wavefront runs a
+    // kind's code at most once a round instead of the first ones running
+    // every kind
+    constexpr int NWV = NT / 64;
+    constexpr int CHV = (NGV * NC + 63) / 64, CHU = (NFG * NC + 63) / 64;
+    const int wv = tid >> 6, ln = tid & 63;
+    auto first_chunk = [&](int g0) { return ((wv - g0) % NWV + NWV) % NWV; };
+    for (int ch = first_chunk(0); ch < CHV; ch += NWV) {
+      const int w = 64 * ch + ln;
+      if (w < NGV * NC) {
         // volume point g, direction UDG component k (volint.m:30-254)
         const int g = w % NGV, k = w / NGV;
         D1 u[NC];
@@ -743,9 +756,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
     };
-    for (int w = tid; w < NGV * NC + NFG * NC; w += NT) {
-      if (w < NGV * NC) continue;
-      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+    for (int ch = first_chunk(CHV); ch < CHU; ch += NWV) {
+      const int i = 64 * ch + ln, t = i % NFG, k = i / NFG;
+      if (i >= NFG * NC) continue;
       double nl[ND];
   #pragma unroll
       for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
@@ -758,9 +771,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       face_point<C>(ap, sAvf[t], nl, u, uh, sBf[t / NGF], fa, fb);
       face_out(t, k, fa, fb);
     }
-    for (int w = tid; w < NGV * NC + NFG * NDIRF; w += NT) {
-      if (w < NGV * NC + NFG * NC) continue;
-      const int i = w - NGV * NC, t = i % NFG, k = i / NFG;
+    for (int ch = first_chunk(CHV + CHU); ch < (NFG * NCU + 63) / 64; ch += NWV) {
+      const int i = NFG * NC + 64 * ch + ln, t = i % NFG, k = i / NFG;
+      if (i >= NFG * NDIRF) continue;
       double nl[ND];
   #pragma unroll
       for (int d = 0; d < ND; ++d) nl[d] = sNl[t + NFG * d];
