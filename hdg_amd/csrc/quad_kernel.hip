@@ -24,6 +24,8 @@
 // shapvg operand is uniform across the wavefront.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "hdgx_device.h"
 #include "hdgx_kernels.h"
 #include "hdgx_shapes.h"
@@ -308,7 +310,10 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
 //     (vj = va + NPV j; a volume node off face j gives 0);
 //   fragment-tile GK: tp | j << 10 | b << 14 | c << 18, the column (va, k)
 //     read at run time from the AE k-step table at tp.
-template <class C, bool FB>
+// SG (face products staged by FaceStage, fragment tiles only): the sums are
+// already in LDS, so F's key is vj | b << 8 | col << 12 | j << 20 (col =
+// c + NCU k) and H's the staged index itself; GK's is unchanged.
+template <class C, bool FB, bool SG = false>
 struct FghKeys {
   using S = S64Shape<C>;
   using FS = FbShape<C>;
@@ -321,9 +326,9 @@ struct FghKeys {
   int32_t f[NF], gk[NG], h[NHK];
 };
 
-template <class C, bool FB>
-constexpr FghKeys<C, FB> make_fgh_keys() {
-  using K = FghKeys<C, FB>;
+template <class C, bool FB, bool SG = false>
+constexpr FghKeys<C, FB, SG> make_fgh_keys() {
+  using K = FghKeys<C, FB, SG>;
   using S = S64Shape<C>;
   constexpr int NPV = K::NPV, NPF = K::NPF, NCU = K::NCU, NDF = K::NDF, NH = K::NH, NB = K::NB;
   constexpr int NGF = K::NGF, NFG = K::NFG;
@@ -350,7 +355,8 @@ constexpr FghKeys<C, FB> make_fgh_keys() {
       const int row = 4 * ri + (ln >> 4), jj = 16 * tb + (ln & 15);
       if (jj >= NH) { k.f[idx] = -1; continue; }
       const int va = row % NPV, c = row / NPV, kk = jj / NDF, bj = jj % NDF, j = bj / NPF, b = bj % NPF;
-      k.f[idx] = (va + NPV * j) | (b << 8) | ((j * NGF + NFG * (c + NCU * kk)) << 12);
+      if constexpr (SG) k.f[idx] = (va + NPV * j) | (b << 8) | ((c + NCU * kk) << 12) | (j << 20);
+      else k.f[idx] = (va + NPV * j) | (b << 8) | ((j * NGF + NFG * (c + NCU * kk)) << 12);
     }
     for (int idx = 0; idx < K::NG; ++idx) {
       const int ln = idx & 63, rest = idx >> 6, rta = rest % S::TRH, s = rest / S::TRH;
@@ -366,15 +372,32 @@ constexpr FghKeys<C, FB> make_fgh_keys() {
       if (jj >= NH) { k.h[idx] = -1; continue; }
       const int c = row % NCU, aj = row / NCU, kk = jj / NDF, bj = jj % NDF;
       const int j = aj / NPF, a = aj % NPF, j2 = bj / NPF, b = bj % NPF;
-      k.h[idx] = j == j2 ? pair(a + NPF * b, j * NGF + NFG * (c + NCU * kk)) : -1;
+      if constexpr (SG) k.h[idx] = j == j2 ? (j * NPF * NPF + a + NPF * b) * NCU * NCU + c + NCU * kk : -1;
+      else k.h[idx] = j == j2 ? pair(a + NPF * b, j * NGF + NFG * (c + NCU * kk)) : -1;
     }
   }
   return k;
 }
 
-template <class C, bool FB>
+template <class C, bool FB, bool SG = false>
 struct FghKeyTab {
-  static constexpr FghKeys<C, FB> tab = make_fgh_keys<C, FB>();
+  static constexpr FghKeys<C, FB, SG> tab = make_fgh_keys<C, FB, SG>();
+};
+
+// The face products of F, GK and H as FP64 MFMA products staged in LDS: per
+// face f, P_f(pair, col) = sum_g shapfgdotshapfc(pair, g) X(f, g, col)
+// (faceint.m:316-359), pair = a + NPF b the face-node pair, X the fhat
+// Jacobian columns col of the array (F: c + NCU k of fhat_u, GK: c + NCU k of
+// fhat_udg, H: c + NCU k of fhat_uh).  Staged at (f NPF^2 + pair) NX + col in
+// the BD product's operand buffer once BD is done: GK alone, then F and H.
+// For the fragment tiles of the 2D orders, where GK's products fit there.
+template <class C>
+struct FaceStage {
+  static constexpr int PT = C::NPF * C::NPF, RTP = (PT + 15) / 16, KSF = (C::NGF + 3) / 4;
+  static constexpr int NXF = C::NCU * C::NCU, NXG = C::NCU * C::NC, NXH = C::NCU * C::NCU;
+  static constexpr int WORDS_GK = C::NFE * PT * NXG, WORDS_FH = C::NFE * PT * (NXF + NXH);
+  static constexpr int CAP = BdShape<C>::KT * BdShape<C>::NCC;
+  static constexpr bool USE = S64Shape<C>::USE && !FbShape<C>::USE && WORDS_GK <= CAP && WORDS_FH <= CAP;
 };
 
 // LDS of one element's quadrature (one workgroup).  RO: residual only (no
@@ -843,7 +866,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     // the gather keys of F, GK and H (FghKeys): F's loaded before the BD
     // products (whose MFMA hides their latency), GK's and H's one array ahead
     constexpr bool FBK = FbShape<C>::USE;
-    using KT_ = FghKeys<C, FBK>;
+    constexpr bool STG = TILE && FaceStage<C>::USE;
+    using KT_ = FghKeys<C, FBK, STG>;
     constexpr int IF = TILE ? (KT_::NF + NT - 1) / NT : 1, IG = TILE ? (KT_::NG + NT - 1) / NT : 1;
     constexpr int IH = TILE ? (KT_::NHK + NT - 1) / NT : 1;
     auto load_keys = [&](const int32_t* key, int nv, int* k, int n) {
@@ -853,7 +877,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     // in registers when they fit (at most 32 a thread), else loaded in the loops
     constexpr bool KPRE = IF + IG + IH <= 32;
     int kf[KPRE ? IF : 1], kg[KPRE ? IG : 1], kh[KPRE ? IH : 1];
-    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK>::tab.f, KT_::NF, kf, IF);
+    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kf, IF);
     using BS = BdShape<C>;
     constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
     double* BD = A.BD + e * (int64_t)NU * C::NCOL;
@@ -879,7 +903,91 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
     }
-    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK>::tab.gk, KT_::NG, kg, IG);
+    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.gk, KT_::NG, kg, IG);
+    if constexpr (STG) {
+      // F, GK and H from the staged face products (FaceStage): one LDS read
+      // an entry
+      using FS = FaceStage<C>;
+      using S = S64Shape<C>;
+      constexpr int PT = FS::PT;
+      double* const sP = sWL;
+      // P_f over the NX columns of X into dst; tasks (face, row tile, column
+      // tile) dealt over the wavefronts
+      auto stage = [&](auto nxc, const double* X, double* dst) {
+        constexpr int NX = decltype(nxc)::value, CB = (NX + 15) / 16, NTASK = NFE * FS::RTP * CB;
+        for (int task = wv; task < NTASK; task += NT / 64) {
+          const int cb = task % CB, rt = (task / CB) % FS::RTP, f = task / (CB * FS::RTP);
+          const int pr = 16 * rt + n, col = 16 * cb + n;
+          d4 acc = {0.0, 0.0, 0.0, 0.0};
+  #pragma unroll
+          for (int s = 0; s < FS::KSF; ++s) {
+            const int k = 4 * s + gq;
+            const double a = (pr < PT && k < NGF) ? sSff[pr + PT * k] : 0.0;
+            const double b = (col < NX && k < NGF) ? X[f * NGF + k + NFG * col] : 0.0;
+            acc = mfma64(a, b, acc);
+          }
+  #pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const int pp = 16 * rt + 4 * i + gq;
+            if (pp < PT && col < NX) dst[(f * PT + pp) * NX + col] = acc[i];
+          }
+        }
+      };
+      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
+      __syncthreads();                        // BD's reads of sWL done
+      stage(std::integral_constant<int, FS::NXG>{}, sGHu, sP);
+      __syncthreads();
+      {
+        // GK: A fragments of the AE product over the face-sparse columns
+        constexpr int NV = S::KSP * S::TRH * 64;
+        const unsigned char* tab = L.sAek;
+        double* GK = A.GK + e * (int64_t)NV;
+  #pragma unroll
+        for (int it = 0; it < IG; ++it) {
+          const int idx = tid + it * NT;
+          const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.gk[idx] : -1);
+          if (idx >= NV) break;
+          const int kcol = kv >= 0 ? tab[kv & 1023] : 255;
+          double v = 0.0;
+          if (kcol != 255) {
+            const int j = (kv >> 10) & 15, b = (kv >> 14) & 15, c = kv >> 18;
+            const int va = kcol % NPV, k = kcol / NPV;
+            const int la = sLoc[va + NPV * j];
+            if (la >= 0) v = sP[(j * PT + la + NPF * b) * FS::NXG + c + NCU * k];
+          }
+          GK[idx] = v;
+        }
+      }
+      __syncthreads();                        // GK's products read
+      stage(std::integral_constant<int, FS::NXF>{}, sFHh, sP);
+      stage(std::integral_constant<int, FS::NXH>{}, sGHh, sP + NFE * PT * FS::NXF);
+      __syncthreads();
+      {
+        // F in the elimination's fragment order
+        constexpr int NV = (NU / 4) * S::TRH * 64;
+        double* F = A.F + e * (int64_t)NV;
+  #pragma unroll
+        for (int it = 0; it < IF; ++it) {
+          const int idx = tid + it * NT;
+          const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
+          if (idx >= NV) break;
+          const int la = kv >= 0 ? sLoc[kv & 255] : -1;
+          F[idx] = la >= 0 ? sP[((kv >> 20) * PT + la + NPF * ((kv >> 8) & 15)) * FS::NXF + ((kv >> 12) & 255)] : 0.0;
+        }
+      }
+      {
+        // H's fragment tiles (block diagonal per face)
+        constexpr int NV = KT_::NHK;
+        const double* sPH = sP + NFE * PT * FS::NXF;
+        double* H = A.H + e * (int64_t)NV;
+  #pragma unroll
+        for (int it = 0; it < IH; ++it) {
+          const int idx = tid + it * NT;
+          const int kv = KPRE ? kh[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.h[idx] : -1);
+          if (idx < NV) H[idx] = kv >= 0 ? sPH[kv] : 0.0;
+        }
+      }
+    }
     // ---- 3d. F (npv,ncu,ncu,ndf) (faceint.m:320,327); F(va, c, k, bj) is u row
     // va + NPV c, trace column k + NCU bj
     auto f_entry = [&](int va, int c, int k, int bj) {
@@ -898,15 +1006,16 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       for (int g = 0; g < NGF; ++g) v += sSff[so + NPF * NPF * g] * X[g];
       return v;
     };
-    if constexpr (TILE && FbShape<C>::USE) {
+    if constexpr (STG) {
+    } else if constexpr (TILE && FbShape<C>::USE) {
       // face blocks: F_f(q, r) = F(u row of q on face f, trace column f NB + r)
       constexpr int NV = (int)FbShape<C>::BLK;
       double* F = A.F + e * (int64_t)NV;
-      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
   #pragma unroll
       for (int it = 0; it < IF; ++it) {
         const int idx = tid + it * NT;
-        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.f[idx] : -1);
+        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
         if (idx < NV) F[idx] = keyed(kv & 255, sFHh + (kv >> 8));
       }
     } else if constexpr (TILE) {
@@ -914,11 +1023,11 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       using S = S64Shape<C>;
       constexpr int NV = (NU / 4) * S::TRH * 64;
       double* F = A.F + e * (int64_t)NV;
-      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK>::tab.h, KT_::NHK, kh, IH);
+      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
   #pragma unroll
       for (int it = 0; it < IF; ++it) {
         const int idx = tid + it * NT;
-        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.f[idx] : -1);
+        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
         if (idx >= NV) break;
         const int la = kv >= 0 ? sLoc[kv & 255] : -1;
         F[idx] = la >= 0 ? keyed(la + NPF * ((kv >> 8) & 15), sFHh + (kv >> 12)) : 0.0;
@@ -945,14 +1054,15 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[la + NPF * b + NPF * NPF * g] * sGHu[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE && FbShape<C>::USE) {
+    if constexpr (STG) {
+    } else if constexpr (TILE && FbShape<C>::USE) {
       // face blocks: GK_f(r, q) = GK(trace row f NB + r, u column of q on face f)
       constexpr int NV = (int)FbShape<C>::BLK;
       double* GK = A.GK + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IG; ++it) {
         const int idx = tid + it * NT;
-        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.gk[idx] : -1);
+        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.gk[idx] : -1);
         if (idx < NV) GK[idx] = keyed(kv & 255, sGHu + (kv >> 8));
       }
     } else if constexpr (TILE) {
@@ -965,7 +1075,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
       for (int it = 0; it < IG; ++it) {
         const int idx = tid + it * NT;
-        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.gk[idx] : -1);
+        const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.gk[idx] : -1);
         if (idx >= NV) break;
         const int kcol = kv >= 0 ? tab[kv & 1023] : 255;
         double v = 0.0;
@@ -997,14 +1107,15 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         for (int g = 0; g < NGF; ++g) v += sSff[a + NPF * b + NPF * NPF * g] * sGHh[j * NGF + g + NFG * (c + NCU * k)];
       return v;
     };
-    if constexpr (TILE) {
+    if constexpr (STG) {
+    } else if constexpr (TILE) {
       // face blocks H_f(r, r') = H(f NB + r, f NB + r'), or fragment tiles
       constexpr int NV = KT_::NHK;
       double* H = A.H + e * (int64_t)NV;
   #pragma unroll
       for (int it = 0; it < IH; ++it) {
         const int idx = tid + it * NT;
-        const int kv = KPRE ? kh[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK>::tab.h[idx] : -1);
+        const int kv = KPRE ? kh[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.h[idx] : -1);
         if (idx < NV) H[idx] = kv >= 0 ? keyed(kv & 255, sGHh + (kv >> 8)) : 0.0;
       }
     } else {
