@@ -183,10 +183,12 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
       if (lane == 0) {
         const double v = sMf[j + NPV * j];
         if (!(v > 0.0)) st |= 1;
-        sMf[j + NPV * j] = sqrt(v);
+        const double r = sqrt(v);
+        sMf[j + NPV * j] = r;
+        sMf[NPV * NPV + j] = 1.0 / r;           // the reciprocal diagonal for the solves
       }
       __syncthreads();
-      if (lane > j && lane < NPV) sMf[j + NPV * lane] /= sMf[j + NPV * j];
+      if (lane > j && lane < NPV) sMf[j + NPV * lane] *= sMf[NPV * NPV + j];
       __syncthreads();
       const int m = NPV - j - 1;
       for (int idx = lane; idx < m * m; idx += 64) {
@@ -209,14 +211,14 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         double s = x[i];
 #pragma unroll
         for (int k = 0; k < i; ++k) s -= sMf[k + NPV * i] * x[k];
-        x[i] = s / sMf[i + NPV * i];
+        x[i] = s * sMf[NPV * NPV + i];
       }
 #pragma unroll
       for (int i = NPV - 1; i >= 0; --i) {
         double s = x[i];
 #pragma unroll
         for (int k = i + 1; k < NPV; ++k) s -= sMf[i + NPV * k] * x[k];
-        x[i] = s / sMf[i + NPV * i];
+        x[i] = s * sMf[NPV * NPV + i];
       }
 #pragma unroll
       for (int i = 0; i < NPV; ++i) dst[i] = x[i];
@@ -327,16 +329,17 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
           if (t >= np) break;
-          // |pr| as an ordered key whose low 6 bits carry 63 - lane: one
-          // max-reduction finds the largest magnitude, ties to the lowest lane
-          unsigned long long key = 0;
-          if (lane < R && !done) key = (Mfma<double>::key(fabs(pr[t])) & ~63ull) | (unsigned long long)(63 - lane);
+          // |pr| as an ordered 32-bit key whose low 6 bits carry 63 - lane:
+          // one max-reduction finds the largest magnitude (piv_key), ties to
+          // the lowest lane
+          unsigned key = 0;
+          if (lane < R && !done) key = piv_key(pr[t], lane);
           key = wave_key_max(key);
           if ((key >> 6) == 0) st |= 2;
           const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63));
           pvt[t] = p;
           const double pv = lane_bcast(pr[t], p);
-          const double ip = 1.0 / pv;
+          const double ip = recip(pv);
           z[t] = (lane == p) ? 1.0 : 0.0;
           // the pivot row's later panel columns and its multipliers of the
           // earlier pivots (its z(t) is 1, later z 0; a row's earlier columns

@@ -18,7 +18,8 @@
 //            row tile rt), so X never leaves the registers.
 //
 // The pivot choice is the one of schur64.hip: the largest |.| of the column
-// among rows not yet pivotal, ties to the lowest row.  Against the other FP64
+// among rows not yet pivotal (to 2^-15 relative, piv_key), ties to the lowest
+// row.  Against the other FP64
 // forms the results agree to rounding.
 #include <hip/hip_runtime.h>
 
@@ -111,14 +112,14 @@ HDGX_DEV void inv64_body(const SchurArgs& A, const int64_t e, const int32_t* per
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
         if (t >= np) break;
-        unsigned long long key = 0;
-        if (!done) key = (Mfma<double>::key(fabs(pr[t])) & ~63ull) | (unsigned long long)(63 - lane);
+        unsigned key = 0;
+        if (!done) key = piv_key(pr[t], lane);
         key = wave_key_max(key);
         if ((key >> 6) == 0) st |= 2;
         const int p = __builtin_amdgcn_readfirstlane(63 - (int)(key & 63));
         pvt[t] = p;
         const double pv = lane_bcast(pr[t], p);
-        const double ip = 1.0 / pv;
+        const double ip = recip(pv);
         z[t] = (lane == p) ? 1.0 : 0.0;
         // the pivot row's later panel columns and its multipliers of the
         // earlier pivots (its z(t) is 1, later z 0; a row's earlier columns

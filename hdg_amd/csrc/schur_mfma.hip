@@ -100,6 +100,25 @@ HDGX_DEV K wave_key_max(K k) {
   return m;
 }
 
+// FP64 pivot-search key of |a| in lane `lane`: the exponent and leading 15
+// mantissa bits of |a| above the low six bits 63 - lane, so one unsigned
+// wavefront max finds the largest magnitude (to 2^-15 relative; rows closer
+// than that count as tied) and ties go to the lowest lane.  NaN counts as
+// infinite.
+HDGX_DEV unsigned piv_key(double a, int lane) {
+  const unsigned long long b = isnan(a) ? 0x7ff0000000000000ull : (unsigned long long)__double_as_longlong(fabs(a));
+  return ((unsigned)(b >> 31) & ~63u) | (unsigned)(63 - lane);
+}
+
+// 1/x to about an ulp: v_rcp_f64 and two Newton steps, a shorter dependent
+// chain than the correctly rounded division
+HDGX_DEV double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // maximum over the wavefront
 HDGX_DEV double wave_max(double v) {
 #pragma unroll
