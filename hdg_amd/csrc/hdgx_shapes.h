@@ -92,20 +92,19 @@ struct FbShape {
 // Per-element words of the element-matrix workspace in hdgx_elemental and the
 // batched launches.  The reference layout keeps every array as
 // kernel/schur_primal.m receives it; the tile layout (FP64 configurations of
-// schur64) stores F, GK and H in the elimination's MFMA fragment order, one
+// schur64) stores F and GK in the elimination's MFMA fragment order, one
 // 64-lane vector per (tile, register) so that every load of the elimination
-// reads 512 contiguous bytes:
+// reads 512 contiguous bytes, and H as its diagonal face blocks:
 //   F  (NU/4, TRH) vectors: lane 16g + c of vector (ri, tb) = F(u row 4 ri + g,
 //      trace column j = 16 tb + c);
 //   GK (KSP, TRH) vectors: lane 16g + c of vector (s, rta) = GK(row 16 rta + c,
 //      column of k-step s, lane group g of the AE table), only the face-sparse
 //      columns (faceint.m:354-358);
-//   H  (TRH, NH/4) vectors: lane 16g + c of vector (tb, rh) = H(row 4 rh + g,
-//      trace column j = 16 tb + c);
-// trace column j = jf + NDF k is H's / F's column k + NCU jf, zero past NH.
-// For FbShape configurations the compact layout is the face-block one:
-//   F  block f: (q, r) at q + NB r;  GK block f: (r, q) at r + NB q;
-//   H  block f: (r, r') at r + NB r'.
+//   H  NFE blocks of NB x NB, NB = NCU NPF (H is block diagonal per face,
+//      faceint.m:347-359): block f holds H(f NB + r, f NB + r') at r + NB r';
+// trace column j = jf + NDF k is F's column k + NCU jf, zero past NH.
+// For FbShape configurations F and GK are face blocks too:
+//   F  block f: (q, r) at q + NB r;  GK block f: (r, q) at r + NB q.
 // The other arrays (BD, Ru, Rh, M, C, E, Rq) keep the reference layout.
 template <class C>
 struct WsLayout {
@@ -122,7 +121,8 @@ struct WsLayout {
   static constexpr size_t gk(bool tile) {
     return tile ? (FB ? FbShape<C>::BLK : (size_t)S::KSP * S::TRH * 64) : NH * NCOL;
   }
-  static constexpr size_t h(bool tile) { return tile ? (FB ? FbShape<C>::BLK : S::TRH * (NH / 4) * 64) : NH * NH; }
+  static constexpr size_t hblk = (size_t)C::NFE * (C::NCU * C::NPF) * (C::NCU * C::NPF);
+  static constexpr size_t h(bool tile) { return tile ? hblk : NH * NH; }
   static constexpr size_t words(bool tile) { return bd + f(tile) + gk(tile) + h(tile) + ru + rh + m + cm + em + rq; }
   static constexpr size_t max_words() {
     return TILE_OK && words(true) > words(false) ? words(true) : words(false);

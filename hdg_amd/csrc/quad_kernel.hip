@@ -304,7 +304,7 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
 // times a face-flux Jacobian entry X(j NGF + g, (c, k)) (faceint.m:316-359),
 // so a key carries the two LDS offsets and the output loops do no index
 // arithmetic.  -1: a structural zero.
-//   face blocks (F, GK, H) and fragment-tile H: so | xo << 8, with so = la +
+//   face blocks (F, GK, H; H in both layouts): so | xo << 8, with so = la +
 //     NPF b the pair's offset and xo = j NGF + NFG (c + NCU k) the column's;
 //   fragment-tile F: vj | b << 8 | xo << 12, la = loc(vj) read at run time
 //     (vj = va + NPV j; a volume node off face j gives 0);
@@ -321,8 +321,7 @@ struct FghKeys {
   static constexpr int NGF = C::NGF, NFG = C::NFE * C::NGF, NB = FS::NB;
   static constexpr int NF = FB ? (int)FS::BLK : (NU / 4) * S::TRH * 64;
   static constexpr int NG = FB ? (int)FS::BLK : S::KSP * S::TRH * 64;
-  static constexpr int NHV = NH / 4;
-  static constexpr int NHK = FB ? (int)FS::BLK : S::TRH * NHV * 64;
+  static constexpr int NHK = C::NFE * NB * NB;   // H's face blocks in both layouts
   int32_t f[NF], gk[NG], h[NHK];
 };
 
@@ -345,10 +344,6 @@ constexpr FghKeys<C, FB, SG> make_fgh_keys() {
       const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, q = rem / NB;
       k.gk[idx] = pair(q % NPF + NPF * (r / NCU), f * NGF + NFG * (r % NCU + NCU * (q / NPF)));
     }
-    for (int idx = 0; idx < K::NHK; ++idx) {
-      const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, r2 = rem / NB;
-      k.h[idx] = pair(r / NCU + NPF * (r2 / NCU), f * NGF + NFG * (r % NCU + NCU * (r2 % NCU)));
-    }
   } else {
     for (int idx = 0; idx < K::NF; ++idx) {
       const int ln = idx & 63, rest = idx >> 6, tb = rest % S::TRH, ri = rest / S::TRH;
@@ -366,15 +361,14 @@ constexpr FghKeys<C, FB, SG> make_fgh_keys() {
       const int c = row % NCU, bj = row / NCU, j = bj / NPF, b = bj % NPF;
       k.gk[idx] = tp | (j << 10) | (b << 14) | (c << 18);
     }
-    for (int idx = 0; idx < K::NHK; ++idx) {
-      const int ln = idx & 63, rest = idx >> 6, rih = rest % K::NHV, tb = rest / K::NHV;
-      const int row = 4 * rih + (ln >> 4), jj = 16 * tb + (ln & 15);
-      if (jj >= NH) { k.h[idx] = -1; continue; }
-      const int c = row % NCU, aj = row / NCU, kk = jj / NDF, bj = jj % NDF;
-      const int j = aj / NPF, a = aj % NPF, j2 = bj / NPF, b = bj % NPF;
-      if constexpr (SG) k.h[idx] = j == j2 ? (j * NPF * NPF + a + NPF * b) * NCU * NCU + c + NCU * kk : -1;
-      else k.h[idx] = j == j2 ? pair(a + NPF * b, j * NGF + NFG * (c + NCU * kk)) : -1;
-    }
+  }
+  // H_f(r, r') = H(f NB + r, f NB + r'): face node a = r / NCU, component
+  // c = r % NCU; b, kk likewise from r'
+  for (int idx = 0; idx < K::NHK; ++idx) {
+    const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, r2 = rem / NB;
+    const int a = r / NCU, c = r % NCU, b = r2 / NCU, kk = r2 % NCU;
+    if constexpr (SG) k.h[idx] = (f * NPF * NPF + a + NPF * b) * NCU * NCU + c + NCU * kk;
+    else k.h[idx] = pair(a + NPF * b, f * NGF + NFG * (c + NCU * kk));
   }
   return k;
 }
@@ -976,7 +970,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
       {
-        // H's fragment tiles (block diagonal per face)
+        // H's diagonal face blocks
         constexpr int NV = KT_::NHK;
         const double* sPH = sP + NFE * PT * FS::NXF;
         double* H = A.H + e * (int64_t)NV;
@@ -1109,7 +1103,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     };
     if constexpr (STG) {
     } else if constexpr (TILE) {
-      // face blocks H_f(r, r') = H(f NB + r, f NB + r'), or fragment tiles
+      // face blocks H_f(r, r') = H(f NB + r, f NB + r')
       constexpr int NV = KT_::NHK;
       double* H = A.H + e * (int64_t)NV;
   #pragma unroll

@@ -126,6 +126,7 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
   constexpr int NQ = UQ ? NPV * NCU * ND : 0;
   constexpr int NRQ = NCU * ND, NCC = NPV * ND, NEE = NDF * ND;
   constexpr int RP = LD::RP, RSTR = LD::RSTR, NCOLP = LD::NCOLP, NHP = LD::NHP;
+  constexpr int NB = NCU * NPF;                 // trace rows of one face
   static_assert(R <= 64, "one panel row per lane");
 
   const int lane = threadIdx.x;
@@ -538,7 +539,8 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         const int k = sStep[16 * rt + 4 * i + g];
         if (k >= 0 && k < NU) sX[k + NCOLP * c] = acc[rt][RT + tb][i];
       }
-    // AE accumulators from H (block diagonal per face on the sparse path)
+    // AE accumulators from H, block diagonal per face (on the sparse path;
+    // the tile layout keeps only the face blocks)
     d4 ae[TRH];
 #pragma unroll
     for (int rta = 0; rta < TRH; ++rta)
@@ -546,11 +548,13 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
       for (int i = 0; i < 4; ++i) {
         const int row = 16 * rta + 4 * i + g;
         double v = 0.0;
-        if constexpr (TILE) {
-          const int rh = 4 * rta + i;
-          if (rh < NH / 4) v = He[(tbg * (NH / 4) + rh) * 64 + lane];
-        } else if (row < NH && j < NH && (!sparse || row / (NCU * NPF) == (j % NDF) / NPF)) {
-          v = He[row + NH * s64_t<C>(j)];
+        if (row < NH && j < NH) {
+          const int t = s64_t<C>(j), fr = row / NB, fc = t / NB;
+          if constexpr (TILE) {
+            if (fr == fc) v = He[fr * NB * NB + (row - fr * NB) + NB * (t - fc * NB)];
+          } else if (!sparse || fr == fc) {
+            v = He[row + NH * t];
+          }
         }
         ae[rta][i] = v;
       }
