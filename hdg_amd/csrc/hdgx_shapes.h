@@ -66,7 +66,10 @@ struct S64Shape {
   // GK's A fragments stay in registers across the RHS tiles when they fit
   // (with three row tiles in the registers D' frees after the elimination)
   static constexpr bool GKREG = KSP * TRH <= (RT == 3 ? 36 : 16);
-  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? 3 : 2);
+  // wavefronts per SIMD the elimination is compiled for: four where the
+  // registers allow it without spills (u-only, two row tiles, at most three
+  // RHS tiles: 2D Euler p = 2)
+  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? (!C::UQ && TRH <= 3 ? 4 : 3) : 2);
 };
 
 // Face-block form of the u-only four-row-tile systems (Euler tetrahedra p = 2:
