@@ -57,6 +57,15 @@ struct QuadThreads {
   static constexpr int N = need() <= 64 ? 64 : 256;
 };
 
+// wavefronts per SIMD the batched quadrature kernel is compiled for: four for
+// 2D Euler (whose single-call kernel fits 128 registers anyway; the batched
+// one would take 8 accumulator registers more and drop to three), three
+// otherwise
+template <class C>
+struct QuadWpe {
+  static constexpr int N = (!C::UQ && C::ND == 2) ? 4 : 3;
+};
+
 typedef Dual<1> D1;
 
 // Loads through the constant address space: with a wavefront-uniform index

@@ -34,7 +34,7 @@ HDGX_DEV int find_seg(const ElemSeg* segs, int nseg, int64_t g) {
 }
 
 template <class C>
-__global__ __launch_bounds__(QuadThreads<C>::N) void quad_seg_kernel(SegTable S, DevMaster T) {
+__global__ __launch_bounds__(QuadThreads<C>::N) __attribute__((amdgpu_waves_per_eu(QuadWpe<C>::N))) void quad_seg_kernel(SegTable S, DevMaster T) {
   constexpr int NPV = C::NPV, NFE = C::NFE, NC = C::NC, NU = C::NU, NH = C::NH, NCOL = C::NCOL;
   constexpr int ND = C::ND, NDF = C::NDF, NCU = C::NCU;
   const int64_t w = blockIdx.x;
