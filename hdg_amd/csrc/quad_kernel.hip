@@ -57,12 +57,14 @@ struct QuadThreads {
   static constexpr int N = need() <= 64 ? 64 : 256;
 };
 
-// wavefronts per SIMD the batched quadrature kernel is compiled for: four for
-// 2D Euler (whose single-call kernel fits 128 registers anyway; the batched
-// one would take 8 accumulator registers more and drop to three), three
-// otherwise
+// wavefronts per SIMD the quadrature kernels are compiled for: the
+// single-call kernel four in 2D (NS p = 3 with its gather keys loaded in the
+// loops; 40 KB of LDS then allows four workgroups per CU), three for
+// tetrahedra; the batched kernel four for 2D Euler (it would otherwise take 8
+// accumulator registers more and drop to three), three otherwise
 template <class C>
 struct QuadWpe {
+  static constexpr int N1 = C::ND == 2 ? 4 : 3;
   static constexpr int N = (!C::UQ && C::ND == 2) ? 4 : 3;
 };
 
@@ -411,7 +413,6 @@ struct QuadLds {
   static constexpr int NGV = C::NGV, NGF = C::NGF, NCU = C::NCU, NC = C::NC, NH = C::NH;
   static constexpr int NR = C::NR, NFG = NFE * NGF;
   static constexpr bool UQ = C::UQ;
-  double sX[NPV * (ND + 1)];
   double sU[NPV * NC];
   double sUH[NH];
   double sSH[NPV * NC];
@@ -419,11 +420,7 @@ struct QuadLds {
   int32_t sLoc[NPV * NFE];
   int32_t sPerm[NPF * NFE];
   int32_t sPermG[NPF * NFE];
-  double sShM[NGV * NPV * (ND + 1)];     // shapmv
-  double sShMF[NGF * NPF * ND];          // shapmf
   unsigned char sAek[RO ? 4 : (S64Shape<C>::TRH * (1 + 4 * S64Shape<C>::KSMAX) + 3) / 4 * 4];
-  double sShV[NGV * NPV];
-  double sShF[NGF * NPF];
   double sSff[NPF * NPF * NGF];
   double sJac[NGV];
   double sXx[NGV * ND * ND];
@@ -437,8 +434,21 @@ struct QuadLds {
   double sUHf[NFG * NCU];
   double sWrk[NR * NCU];
   // the B rows of BD's product (BdShape): volume rows wrl (r < NR), then the
-  // face rows fhat_udg (NR + t), each row's NCU*NC columns contiguous
-  double sWL[RO ? 1 : BdShape<C>::KT * BdShape<C>::NCC + 16];
+  // face rows fhat_udg (NR + t), each row's NCU*NC columns contiguous.  They
+  // are first written by the physics phase, so they share their storage with
+  // the tables only the staging and pointwise phases read (the element's
+  // coordinates, shapmv, shapmf and the interpolation tables).
+  struct Ph1 {
+    double sX[NPV * (ND + 1)];
+    double sShM[NGV * NPV * (ND + 1)];     // shapmv
+    double sShMF[NGF * NPF * ND];          // shapmf
+    double sShV[NGV * NPV];
+    double sShF[NGF * NPF];
+  };
+  union {
+    Ph1 ph1;
+    double sWL[RO ? 1 : BdShape<C>::KT * BdShape<C>::NCC + 16];
+  };
   double sFH[NFG * NCU];
   double sFHh[RO ? 1 : NFG * NCU * NCU];
   double sGH[NFG * NCU];
@@ -469,14 +479,14 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   const int ncd = ap.ncd;
   const bool tdep = ap.tdep != 0;
 
-  double* const sX = L.sX;
+  double* const sX = L.ph1.sX;
   double* const sU = L.sU;
   double* const sUH = L.sUH;
   double* const sSH = L.sSH;
   int32_t* const sBf = L.sBf;
   int32_t* const sLoc = L.sLoc;
-  double* const sShV = L.sShV;
-  double* const sShF = L.sShF;
+  double* const sShV = L.ph1.sShV;
+  double* const sShF = L.ph1.sShF;
   double* const sSff = L.sSff;
   double* const sJac = L.sJac;
   double* const sXx = L.sXx;
@@ -529,8 +539,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     const int32_t ak = (aek_on && tid < NAEK) ? reinterpret_cast<const int32_t*>(T.aek[UQ ? 1 : 0])[tid] : 0;
 #pragma unroll
     for (int it = 0; it < NMVI; ++it)
-      if (tid + it * NT < NMV) L.sShM[tid + it * NT] = mv[it];
-    if (tid < NGF * NPF * ND) L.sShMF[tid] = mf;
+      if (tid + it * NT < NMV) L.ph1.sShM[tid + it * NT] = mv[it];
+    if (tid < NGF * NPF * ND) L.ph1.sShMF[tid] = mf;
     if (tid < NPF * NFE) { L.sPerm[tid] = pm; L.sPermG[tid] = pg; }
     if (aek_on && tid < NAEK) reinterpret_cast<int32_t*>(L.sAek)[tid] = ak;
     if (tid < NPV * ncd) sX[tid] = x;
@@ -585,7 +595,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         // volume geometry: jac = det J, Xx(a, b) = jac * dxi_b/dx_a
         const int g = w;
         double Xx[ND][ND];
-        const double jac = vol_geom<C>(L.sShM, sX, g, Xx);
+        const double jac = vol_geom<C>(L.ph1.sShM, sX, g, Xx);
         sJac[g] = jac;
 #pragma unroll
         for (int a = 0; a < ND; ++a)
@@ -593,20 +603,20 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           for (int b = 0; b < ND; ++b) sXx[g * ND * ND + a * ND + b] = Xx[a][b];
         double av = 0.0;
         if (ap.av_col >= 0)
-          for (int a = 0; a < NPV; ++a) av += L.sShM[g + NGV * a] * sX[a + NPV * ap.av_col];
+          for (int a = 0; a < NPV; ++a) av += L.ph1.sShM[g + NGV * a] * sX[a + NPV * ap.av_col];
         sAv[g] = av;
       } else {
         // face geometry: unit normal and surface Jacobian
         const int t = w - NGV;
         const int j = t / NGF, g = t - j * NGF;
         double nl[ND];
-        const double jac = face_geom<C>(L.sShMF, L.sPermG, sX, j, g, nl);
+        const double jac = face_geom<C>(L.ph1.sShMF, L.sPermG, sX, j, g, nl);
 #pragma unroll
         for (int i = 0; i < ND; ++i) sNl[t + NFG * i] = nl[i];
         sFJ[t] = jac;
         double av = 0.0;
         if (ap.av_col >= 0)
-          for (int a = 0; a < NPF; ++a) av += L.sShMF[g + NGF * a] * sX[L.sPermG[a + NPF * j] + NPV * ap.av_col];
+          for (int a = 0; a < NPF; ++a) av += L.ph1.sShMF[g + NGF * a] * sX[L.sPermG[a + NPF * j] + NPV * ap.av_col];
         sAvf[t] = av;
       }
     }
@@ -877,8 +887,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
       for (int it = 0; it < n; ++it) k[it] = tid + it * NT < nv ? key[tid + it * NT] : -1;
     };
-    // in registers when they fit (at most 32 a thread), else loaded in the loops
-    constexpr bool KPRE = IF + IG + IH <= 32;
+    // in registers when they fit (at most 16 a thread), else loaded in the
+    // loops (NS p = 3: 20, which would cost the fourth wavefront per SIMD)
+    constexpr bool KPRE = IF + IG + IH <= 16;
     int kf[KPRE ? IF : 1], kg[KPRE ? IG : 1], kh[KPRE ? IH : 1];
     if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kf, IF);
     using BS = BdShape<C>;
@@ -1135,7 +1146,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
 }
 
 template <class C, bool TILE = false>
-__global__ __launch_bounds__(QuadThreads<C>::N) __attribute__((amdgpu_waves_per_eu(3))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
+__global__ __launch_bounds__(QuadThreads<C>::N) __attribute__((amdgpu_waves_per_eu(QuadWpe<C>::N1))) void quad_kernel(QuadArgs A, DevMaster T, DevApp ap) {
   __shared__ QuadLds<C> lds;
   quad_body<C, false, TILE, QuadThreads<C>::N>(A, T, ap, blockIdx.x, lds);
 }
