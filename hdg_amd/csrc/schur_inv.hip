@@ -41,7 +41,8 @@ struct InvShape {
   static constexpr int RP = R + 1, RSTR = 16 * RT + 1;
   static constexpr int LDI = NU + 1;              // staging of T, odd leading dimension
   static constexpr int NSCR = 8 * RP + 4 * RSTR;
-  static constexpr int NST = NU * LDI > NSCR ? NU * LDI : NSCR;
+  // T is staged 32 of its columns at a time (two column tiles)
+  static constexpr int NST = 32 * LDI > NSCR ? 32 * LDI : NSCR;
 };
 
 template <class C>
@@ -216,25 +217,30 @@ HDGX_DEV void inv64_body(const SchurArgs& A, const int64_t e, const int32_t* per
         if (k >= 0 && k < NU) L.sDu[k] = acc[rt][CTD][i];
       }
   }
-  double* const sT = L.sX;                     // T(r, k) at r + LDI k
-#pragma unroll
-  for (int rt = 0; rt < RT; ++rt)
-#pragma unroll
-    for (int ct = 0; ct < RT; ++ct)
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int row = 16 * rt + 4 * i + g, col = 16 * ct + c;
-        if (row < NU && col < NU) sT[row + LDI * col] = acc[rt][ct][i];
-      }
-  __syncthreads();
-  // D^-1(a, b) = T(p_a, column holding T's column b) = sT[p_a + LDI sStep[b]],
-  // written over D
+  // D^-1(a, b) = T(p_a, column holding T's column b) = T(p_a, sStep[b]),
+  // written over D; T staged in LDS 32 columns at a time
   // (a singular D, status bit 2, can leave a row unpivoted or pivot a padding
   // row: those entries become NaN rather than read past T)
-  for (int idx = lane; idx < NU * NU; idx += 64) {
-    const int a = idx % NU, b = idx / NU;
-    const int pa = L.sP[a], kb = L.sStep[b];
-    BDe[idx] = (pa >= 0 && pa < NU && kb >= 0) ? sT[pa + LDI * kb] : __builtin_nan("");
+  double* const sT = L.sX;                     // T(r, 32 h + k) at r + LDI k
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+      for (int ct = 2 * h; ct < 2 * h + 2; ++ct)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int row = 16 * rt + 4 * i + g, col = 16 * ct + c;
+          if (row < NU && col < NU) sT[row + LDI * (col - 32 * h)] = acc[rt][ct][i];
+        }
+    __syncthreads();
+    for (int idx = lane; idx < NU * NU; idx += 64) {
+      const int a = idx % NU, b = idx / NU;
+      const int pa = L.sP[a], kb = L.sStep[b];
+      if ((h == 0) != (kb < 32)) continue;     // the other half's column (kb < 0: the first)
+      BDe[idx] = (pa >= 0 && pa < NU && kb >= 0) ? sT[pa + LDI * (kb - 32 * h)] : __builtin_nan("");
+    }
+    __syncthreads();
   }
   const int LDO = C::NPV * A.nco;
   for (int r = lane; r < LDO; r += 64) {
@@ -530,7 +536,7 @@ HDGX_DEV void apply64_fb_body(const SchurArgs& A, const int64_t e, const int fc,
 }
 
 template <class C, bool FB>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void inv64_kernel(SchurArgs A,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void inv64_kernel(SchurArgs A,
                                                                                              const int32_t* perm) {
   __shared__ InvLds<C> lds;
   inv64_body<C, FB>(A, blockIdx.x, perm, lds);

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>()
 // the four-row-tile u-only configurations: D^-1 over the workspace's D, then
 // the MFMA products (schur_inv.hip), as hdgx_elemental does
 template <class C>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void inv_seg_kernel(SegTable S,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void inv_seg_kernel(SegTable S,
                                                                                                const int32_t* perm) {
   if constexpr (InvShape<C>::USE) {
     const int64_t w = blockIdx.x;
