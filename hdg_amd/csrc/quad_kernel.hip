@@ -891,7 +891,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
     // loops (NS p = 3: 20, which would cost the fourth wavefront per SIMD)
     constexpr bool KPRE = IF + IG + IH <= 16;
     int kf[KPRE ? IF : 1], kg[KPRE ? IG : 1], kh[KPRE ? IH : 1];
-    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kf, IF);
+    if constexpr (TILE && KPRE && !STG) load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kf, IF);
     using BS = BdShape<C>;
     constexpr int KSB = BS::KSB, KT = BS::KT, NVV = BS::NVV;
     double* BD = A.BD + e * (int64_t)NU * C::NCOL;
@@ -917,7 +917,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         }
       }
     }
-    if constexpr (TILE && KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.gk, KT_::NG, kg, IG);
+    if constexpr (TILE && KPRE && !STG) load_keys(FghKeyTab<C, FBK, STG>::tab.gk, KT_::NG, kg, IG);
     if constexpr (STG) {
       // F, GK and H from the staged face products (FaceStage): one LDS read
       // an entry
@@ -947,7 +947,10 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           }
         }
       };
-      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
+      // each copy loop's keys, loaded ahead of its staging products and
+      // barriers, which hide their latency
+      int kgs[IG];
+      load_keys(FghKeyTab<C, FBK, STG>::tab.gk, KT_::NG, kgs, IG);
       __syncthreads();                        // BD's reads of sWL done
       stage(std::integral_constant<int, FS::NXG>{}, sGHu, sP);
       __syncthreads();
@@ -959,7 +962,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
         for (int it = 0; it < IG; ++it) {
           const int idx = tid + it * NT;
-          const int kv = KPRE ? kg[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.gk[idx] : -1);
+          const int kv = kgs[it];
           if (idx >= NV) break;
           const int kcol = kv >= 0 ? tab[kv & 1023] : 255;
           double v = 0.0;
@@ -972,6 +975,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
           GK[idx] = v;
         }
       }
+      int kfs[IF], khs[IH];
+      load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kfs, IF);
+      load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, khs, IH);
       __syncthreads();                        // GK's products read
       stage(std::integral_constant<int, FS::NXF>{}, sFHh, sP);
       stage(std::integral_constant<int, FS::NXH>{}, sGHh, sP + NFE * PT * FS::NXF);
@@ -983,7 +989,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
         for (int it = 0; it < IF; ++it) {
           const int idx = tid + it * NT;
-          const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
+          const int kv = kfs[it];
           if (idx >= NV) break;
           const int la = kv >= 0 ? sLoc[kv & 255] : -1;
           F[idx] = la >= 0 ? sP[((kv >> 20) * PT + la + NPF * ((kv >> 8) & 15)) * FS::NXF + ((kv >> 12) & 255)] : 0.0;
@@ -997,7 +1003,7 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
   #pragma unroll
         for (int it = 0; it < IH; ++it) {
           const int idx = tid + it * NT;
-          const int kv = KPRE ? kh[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.h[idx] : -1);
+          const int kv = khs[it];
           if (idx < NV) H[idx] = kv >= 0 ? sPH[kv] : 0.0;
         }
       }
