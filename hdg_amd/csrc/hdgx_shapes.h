@@ -66,10 +66,14 @@ struct S64Shape {
   // GK's A fragments stay in registers across the RHS tiles when they fit
   // (with three row tiles in the registers D' frees after the elimination)
   static constexpr bool GKREG = KSP * TRH <= (RT == 3 ? 36 : 16);
-  // wavefronts per SIMD the elimination is compiled for: four where the
-  // registers allow it without spills (u-only, two row tiles, at most three
-  // RHS tiles: 2D Euler p = 2)
-  static constexpr int WPE = RT == 1 ? 4 : (RT == 2 ? (!C::UQ && TRH <= 3 ? 4 : 3) : 2);
+  // wavefronts per SIMD the elimination is compiled for, as many as the
+  // registers allow without spills: the tile-layout kernel (WPE), the batched
+  // one (WPE_SEG) and the reference-layout one (WPE_REF); u-only systems of
+  // one row tile (2D Euler p = 1) 8, 6 and 4; four for u-only two-row-tile
+  // systems with at most three RHS tiles (2D Euler p = 2)
+  static constexpr int WPE = RT == 1 ? (C::UQ ? 4 : 8) : (RT == 2 ? (!C::UQ && TRH <= 3 ? 4 : 3) : 2);
+  static constexpr int WPE_SEG = RT == 1 && !C::UQ ? 6 : WPE;
+  static constexpr int WPE_REF = RT == 1 ? 4 : WPE;
 };
 
 // Face-block form of the u-only four-row-tile systems (Euler tetrahedra p = 2:

@@ -706,7 +706,8 @@ __host__ __device__ constexpr int64_t s64_blocks(int64_t ne) {
 }
 
 template <class C, bool TILE>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(S64Shape<C>::WPE, S64Shape<C>::WPE))) void schur64_kernel(SchurArgs A) {
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(TILE ? S64Shape<C>::WPE : S64Shape<C>::WPE_REF,
+                                                                   TILE ? S64Shape<C>::WPE : S64Shape<C>::WPE_REF))) void schur64_kernel(SchurArgs A) {
   __shared__ S64Lds<C> lds;
   constexpr int NW = S64Shape<C>::NW;
   if constexpr (NW == 1) {

@@ -113,7 +113,7 @@ HDGX_DEV SchurArgs seg_schur_args(const SegTable& S, int64_t w) {
 
 // the elimination body of hdgx_elemental's FP64 path for this configuration
 template <class C>
-constexpr int seg_wpe() { return S64Shape<C>::USE ? S64Shape<C>::WPE : MfmaShape<C>::WPE; }
+constexpr int seg_wpe() { return S64Shape<C>::USE ? S64Shape<C>::WPE_SEG : MfmaShape<C>::WPE; }
 
 template <class C>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(seg_wpe<C>(), seg_wpe<C>()))) void schur_seg_kernel(SegTable S, const int32_t* __restrict__ perm, const unsigned char* __restrict__ aek) {
