@@ -888,7 +888,8 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       for (int it = 0; it < n; ++it) k[it] = tid + it * NT < nv ? key[tid + it * NT] : -1;
     };
     // in registers when they fit (at most 16 a thread), else loaded in the
-    // loops (NS p = 3: 20, which would cost the fourth wavefront per SIMD)
+    // loops; the staged path (STG) loads each copy loop's keys itself, ahead
+    // of its staging products
     constexpr bool KPRE = IF + IG + IH <= 16;
     int kf[KPRE ? IF : 1], kg[KPRE ? IG : 1], kh[KPRE ? IH : 1];
     if constexpr (TILE && KPRE && !STG) load_keys(FghKeyTab<C, FBK, STG>::tab.f, KT_::NF, kf, IF);
