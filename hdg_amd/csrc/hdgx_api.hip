@@ -173,8 +173,16 @@ int pack_master(hdgx_ctx* ctx, const hdgx_master* m, MasterImage* img) {
   for (size_t i = 0; i < n_perm; ++i)
     if (m->permgeom[i] < 0 || m->permgeom[i] >= npv) return fail(ctx, HDGX_ERR_ARG, "master: permgeom out of range");
   const int ncu = nd + 2;
-  const std::vector<unsigned char> aek_u = ae_table(npv, npf, nfe, ncu, ncu, m->perm);
-  const std::vector<unsigned char> aek_uq = ae_table(npv, npf, nfe, ncu, ncu * (nd + 1), m->perm);
+  std::vector<unsigned char> aek_u = ae_table(npv, npf, nfe, ncu, ncu, m->perm);
+  std::vector<unsigned char> aek_uq = ae_table(npv, npf, nfe, ncu, ncu * (nd + 1), m->perm);
+  // after each AE table the node-on-face table loc(v, j) as bytes (255: v is
+  // not on face j), which the elimination reads with it to gather F's face
+  // blocks
+  for (size_t i = 0; i < n_loc; ++i) {
+    const unsigned char b = loc[i] < 0 ? (unsigned char)255 : (unsigned char)loc[i];
+    aek_u.push_back(b);
+    aek_uq.push_back(b);
+  }
   const std::vector<double> bdv = bd_table(m, loc);
   // every table starts on a 16-byte boundary
   img->host.clear();
@@ -355,8 +363,8 @@ int ensure_ws(hdgx_ctx* ctx, size_t bytes) {
 // hdgx_elemental's element loop over chunks of `chunk` elements, the element
 // matrices of each chunk in the workspace at ws (ws_words_of * chunk
 // doubles): the quadrature kernel writes them, the elimination reads them.
-// The FP64 eliminations of up to three row tiles take F, GK and H in their
-// fragment order (the tile layout of hdgx_shapes.h).
+// The FP64 eliminations of up to three row tiles take F, GK and H in the tile
+// layout of hdgx_shapes.h.
 int elemental_launches(hdgx_ctx* ctx, const MasterBuf& mb, const DevApp& ap, const hdgx_master* master,
                        const hdgx_app* app, int64_t ne, const double* dgnodes, const int32_t* bf, const double* udg,
                        const double* uh, const double* sh, double* ae, double* fe, double* dudg, double* dudg_duh,

@@ -18,8 +18,8 @@
 
 namespace hdgx {
 
-// tile: write F, GK and H in the elimination's fragment order (WsLayout in
-// hdgx_shapes.h) for a schur_launch with tile set; otherwise the reference layout
+// tile: write F, GK and H in the tile layout (WsLayout in hdgx_shapes.h) for a
+// schur_launch with tile set; otherwise the reference layout
 template <class C> hipError_t quad_launch(hipStream_t s, const QuadArgs& a, const DevMaster& t, const DevApp& ap,
                                           bool tile);
 // residual only (Ru, Rh with hdg_residual's sign, Rq)
@@ -32,7 +32,7 @@ template <class C> hipError_t getq_launch(hipStream_t s, const GetqArgs& a, cons
 // prec 0: FP64 MFMA Gauss-Jordan; 1: FP32 MFMA Gauss-Jordan (with D'^-1) + FP64
 // refinement.
 // perm (device, may be null) marks GK as face sparse; tile: F, GK and H in the
-// fragment order (FP64, configurations with WsLayout<C>::TILE_OK)
+// tile layout (FP64, configurations with WsLayout<C>::TILE_OK)
 template <class C> hipError_t schur_launch(hipStream_t s, const SchurArgs& a, const int32_t* perm, bool tile);
 // per-element words of the element-matrix workspace (the larger of its two layouts)
 template <class C> size_t ws_words();

@@ -99,19 +99,20 @@ struct FbShape {
 // Per-element words of the element-matrix workspace in hdgx_elemental and the
 // batched launches.  The reference layout keeps every array as
 // kernel/schur_primal.m receives it; the tile layout (FP64 configurations of
-// schur64) stores F and GK in the elimination's MFMA fragment order, one
-// 64-lane vector per (tile, register) so that every load of the elimination
-// reads 512 contiguous bytes, and H as its diagonal face blocks:
-//   F  (NU/4, TRH) vectors: lane 16g + c of vector (ri, tb) = F(u row 4 ri + g,
-//      trace column j = 16 tb + c);
+// schur64) stores F and H as face blocks and GK in the elimination's MFMA
+// fragment order, one 64-lane vector per (tile, register) so that every GK
+// load of the elimination reads 512 contiguous bytes (NB = NCU NPF):
+//   F  NFE blocks of NB x NB (F couples a face's trace nodes only with the
+//      volume nodes on that face, faceint.m:316-327): block f holds F(u row
+//      of q on face f, trace column f NB + r) at q + NB r, with q = a + NPF m
+//      (face node a, i.e. volume node perm(a, f), component m) and r = k +
+//      NCU b (component k, face node b);
 //   GK (KSP, TRH) vectors: lane 16g + c of vector (s, rta) = GK(row 16 rta + c,
 //      column of k-step s, lane group g of the AE table), only the face-sparse
 //      columns (faceint.m:354-358);
-//   H  NFE blocks of NB x NB, NB = NCU NPF (H is block diagonal per face,
-//      faceint.m:347-359): block f holds H(f NB + r, f NB + r') at r + NB r';
-// trace column j = jf + NDF k is F's column k + NCU jf, zero past NH.
-// For FbShape configurations F and GK are face blocks too:
-//   F  block f: (q, r) at q + NB r;  GK block f: (r, q) at r + NB q.
+//   H  NFE blocks of NB x NB (H is block diagonal per face,
+//      faceint.m:347-359): block f holds H(f NB + r, f NB + r') at r + NB r'.
+// For FbShape configurations GK is face blocks too: block f: (r, q) at r + NB q.
 // The other arrays (BD, Ru, Rh, M, C, E, Rq) keep the reference layout.
 template <class C>
 struct WsLayout {
@@ -122,9 +123,8 @@ struct WsLayout {
   static constexpr size_t bd = NU * NCOL, ru = NU, rh = NH;
   static constexpr size_t m = C::UQ ? NPV * NPV : 0, cm = C::UQ ? NPV * NPV * ND : 0;
   static constexpr size_t em = C::UQ ? NPV * C::NDF * ND : 0, rq = C::UQ ? NPV * C::NCU * ND : 0;
-  static constexpr size_t f(bool tile) {
-    return tile ? (FB ? FbShape<C>::BLK : (NU / 4) * S::TRH * 64) : NU * NH;
-  }
+  static constexpr size_t fblk = (size_t)C::NFE * (C::NCU * C::NPF) * (C::NCU * C::NPF);
+  static constexpr size_t f(bool tile) { return tile ? fblk : NU * NH; }
   static constexpr size_t gk(bool tile) {
     return tile ? (FB ? FbShape<C>::BLK : (size_t)S::KSP * S::TRH * 64) : NH * NCOL;
   }

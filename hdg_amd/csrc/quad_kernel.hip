@@ -315,22 +315,21 @@ HDGX_DEV void face_point(const DevApp& ap, double av, const double* nl, const TU
 // times a face-flux Jacobian entry X(j NGF + g, (c, k)) (faceint.m:316-359),
 // so a key carries the two LDS offsets and the output loops do no index
 // arithmetic.  -1: a structural zero.
-//   face blocks (F, GK, H; H in both layouts): so | xo << 8, with so = la +
-//     NPF b the pair's offset and xo = j NGF + NFG (c + NCU k) the column's;
-//   fragment-tile F: vj | b << 8 | xo << 12, la = loc(vj) read at run time
-//     (vj = va + NPV j; a volume node off face j gives 0);
+//   face blocks (F and H in both layouts, GK in FbShape's): so | xo << 8,
+//     with so = la + NPF b the pair's offset and xo = j NGF + NFG (c + NCU k)
+//     the column's;
 //   fragment-tile GK: tp | j << 10 | b << 14 | c << 18, the column (va, k)
 //     read at run time from the AE k-step table at tp.
 // SG (face products staged by FaceStage, fragment tiles only): the sums are
-// already in LDS, so F's key is vj | b << 8 | col << 12 | j << 20 (col =
-// c + NCU k) and H's the staged index itself; GK's is unchanged.
+// already in LDS, so F's and H's keys are the staged indices themselves;
+// GK's is unchanged.
 template <class C, bool FB, bool SG = false>
 struct FghKeys {
   using S = S64Shape<C>;
   using FS = FbShape<C>;
   static constexpr int NPV = C::NPV, NPF = C::NPF, NCU = C::NCU, NDF = C::NDF, NH = C::NH, NU = C::NU;
   static constexpr int NGF = C::NGF, NFG = C::NFE * C::NGF, NB = FS::NB;
-  static constexpr int NF = FB ? (int)FS::BLK : (NU / 4) * S::TRH * 64;
+  static constexpr int NF = C::NFE * NB * NB;    // F's face blocks in both layouts
   static constexpr int NG = FB ? (int)FS::BLK : S::KSP * S::TRH * 64;
   static constexpr int NHK = C::NFE * NB * NB;   // H's face blocks in both layouts
   int32_t f[NF], gk[NG], h[NHK];
@@ -344,26 +343,20 @@ constexpr FghKeys<C, FB, SG> make_fgh_keys() {
   constexpr int NGF = K::NGF, NFG = K::NFG;
   K k{};
   auto pair = [](int so, int xo) { return so | (xo << 8); };
+  // F_f(q, r): q = a + NPF m (face node a of face f, component m), r = kk + NCU b
+  for (int idx = 0; idx < K::NF; ++idx) {
+    const int f = idx / (NB * NB), rem = idx % (NB * NB), q = rem % NB, r = rem / NB;
+    const int a = q % NPF, m = q / NPF, kk = r % NCU, b = r / NCU;
+    if constexpr (SG) k.f[idx] = (f * NPF * NPF + a + NPF * b) * NCU * NCU + m + NCU * kk;
+    else k.f[idx] = pair(a + NPF * b, f * NGF + NFG * (m + NCU * kk));
+  }
   if constexpr (FB) {
-    for (int idx = 0; idx < K::NF; ++idx) {
-      // F_f(q, r): q = a + NPF m (face node a of face f, component m), r = kk + NCU b
-      const int f = idx / (NB * NB), rem = idx % (NB * NB), q = rem % NB, r = rem / NB;
-      k.f[idx] = pair(q % NPF + NPF * (r / NCU), f * NGF + NFG * (q / NPF + NCU * (r % NCU)));
-    }
     for (int idx = 0; idx < K::NG; ++idx) {
       // GK_f(r, q): trace row r = c + NCU b, volume q = a + NPF kk
       const int f = idx / (NB * NB), rem = idx % (NB * NB), r = rem % NB, q = rem / NB;
       k.gk[idx] = pair(q % NPF + NPF * (r / NCU), f * NGF + NFG * (r % NCU + NCU * (q / NPF)));
     }
   } else {
-    for (int idx = 0; idx < K::NF; ++idx) {
-      const int ln = idx & 63, rest = idx >> 6, tb = rest % S::TRH, ri = rest / S::TRH;
-      const int row = 4 * ri + (ln >> 4), jj = 16 * tb + (ln & 15);
-      if (jj >= NH) { k.f[idx] = -1; continue; }
-      const int va = row % NPV, c = row / NPV, kk = jj / NDF, bj = jj % NDF, j = bj / NPF, b = bj % NPF;
-      if constexpr (SG) k.f[idx] = (va + NPV * j) | (b << 8) | ((c + NCU * kk) << 12) | (j << 20);
-      else k.f[idx] = (va + NPV * j) | (b << 8) | ((j * NGF + NFG * (c + NCU * kk)) << 12);
-    }
     for (int idx = 0; idx < K::NG; ++idx) {
       const int ln = idx & 63, rest = idx >> 6, rta = rest % S::TRH, s = rest / S::TRH;
       const int row = 16 * rta + (ln & 15);
@@ -984,16 +977,14 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       stage(std::integral_constant<int, FS::NXH>{}, sGHh, sP + NFE * PT * FS::NXF);
       __syncthreads();
       {
-        // F in the elimination's fragment order
-        constexpr int NV = (NU / 4) * S::TRH * 64;
+        // F's face blocks
+        constexpr int NV = KT_::NF;
         double* F = A.F + e * (int64_t)NV;
   #pragma unroll
         for (int it = 0; it < IF; ++it) {
           const int idx = tid + it * NT;
           const int kv = kfs[it];
-          if (idx >= NV) break;
-          const int la = kv >= 0 ? sLoc[kv & 255] : -1;
-          F[idx] = la >= 0 ? sP[((kv >> 20) * PT + la + NPF * ((kv >> 8) & 15)) * FS::NXF + ((kv >> 12) & 255)] : 0.0;
+          if (idx < NV) F[idx] = sP[kv];
         }
       }
       {
@@ -1028,9 +1019,9 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
       return v;
     };
     if constexpr (STG) {
-    } else if constexpr (TILE && FbShape<C>::USE) {
+    } else if constexpr (TILE) {
       // face blocks: F_f(q, r) = F(u row of q on face f, trace column f NB + r)
-      constexpr int NV = (int)FbShape<C>::BLK;
+      constexpr int NV = KT_::NF;
       double* F = A.F + e * (int64_t)NV;
       if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
   #pragma unroll
@@ -1038,20 +1029,6 @@ HDGX_DEV void quad_body(const QuadArgs& A, const DevMaster& T, const DevApp& ap,
         const int idx = tid + it * NT;
         const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
         if (idx < NV) F[idx] = keyed(kv & 255, sFHh + (kv >> 8));
-      }
-    } else if constexpr (TILE) {
-      // the elimination's fragment order (hdgx_shapes.h, WsLayout)
-      using S = S64Shape<C>;
-      constexpr int NV = (NU / 4) * S::TRH * 64;
-      double* F = A.F + e * (int64_t)NV;
-      if constexpr (KPRE) load_keys(FghKeyTab<C, FBK, STG>::tab.h, KT_::NHK, kh, IH);
-  #pragma unroll
-      for (int it = 0; it < IF; ++it) {
-        const int idx = tid + it * NT;
-        const int kv = KPRE ? kf[KPRE ? it : 0] : (idx < NV ? FghKeyTab<C, FBK, STG>::tab.f[idx] : -1);
-        if (idx >= NV) break;
-        const int la = kv >= 0 ? sLoc[kv & 255] : -1;
-        F[idx] = la >= 0 ? keyed(la + NPF * ((kv >> 8) & 15), sFHh + (kv >> 12)) : 0.0;
       }
     } else {
       double* F = A.F + e * (int64_t)NU * NH;

@@ -101,15 +101,17 @@ struct S64Lds {
   double sMiC[UQ ? NPV * NPV * C::ND : 1];
   double sMiE[UQ ? NPV * C::NDF * C::ND : 1];
   int sStep[R];
-  unsigned char sTab[(NTAB + 7) / 8 * 8];        // klen[TRH] then the k-step table
+  // klen[TRH], the k-step table, then (tile layout) the node-on-face table
+  // loc(v, f) of F's face blocks
+  unsigned char sTab[(NTAB + C::NPV * C::NFE + 7) / 8 * 8];
 };
 
-// Element e of A (one 64-lane wavefront), LDS in L.  TILE: F, GK and H are
-// in the fragment order of WsLayout (hdgx_shapes.h) and A.aek is the master's
-// AE table; otherwise every array has the reference layout.  With four row
-// tiles the RHS tiles are split over S::NW wavefronts: `part` takes RHS tiles
-// part*TRW .. part*TRW + TRW - 1 (AE / DUDG_DUH columns), part 0 also du,
-// DUDG and FE; the status word is OR-ed into A.status[e], which the launch
+// Element e of A (one 64-lane wavefront), LDS in L.  TILE: F and H are face
+// blocks and GK is in fragment order (WsLayout, hdgx_shapes.h), and A.aek is
+// the master's AE table followed by its node-on-face table; otherwise every
+// array has the reference layout.  With four row tiles the RHS tiles are split
+// over S::NW wavefronts: `part` takes RHS tiles part*TRW .. part*TRW + TRW - 1
+// (AE / DUDG_DUH columns), part 0 also du, DUDG and FE; the status word is OR-ed into A.status[e], which the launch
 // zeroes first.
 template <class C, bool TILE = false>
 HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, S64Lds<C>& L) {
@@ -159,10 +161,22 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         }
         acc[rt][ct][i] = v;
       }
+  // F's face blocks (tile layout), read whole with coalesced loads and put in
+  // sX past the u,q prologue's Cholesky factor, where the -F gather below
+  // reads them
+  constexpr int NFB = C::NFE * NB * NB, FOFF = UQ ? NPV * NPV + NPV : 0;
+  constexpr bool FST = TILE;
+  static_assert(!TILE || FOFF + NFB <= LD::NSX, "F's face blocks fit in sX");
+  constexpr int IFS = FST ? (NFB + 63) / 64 : 1;
+  double fr[IFS];
+  if constexpr (FST) {
+#pragma unroll
+    for (int it = 0; it < IFS; ++it) fr[it] = lane + 64 * it < NFB ? Fe[lane + 64 * it] : 0.0;
+  }
 
   // ---- AE k-step table: the face-sparse one of the master, or every column
   if (sparse) {
-    for (int i = lane; i < LD::NTAB; i += 64) L.sTab[i] = A.aek[i];
+    for (int i = lane; i < LD::NTAB + (TILE ? NPV * C::NFE : 0); i += 64) L.sTab[i] = A.aek[i];
   } else {
     for (int i = lane; i < LD::NTAB; i += 64) {
       int v;
@@ -225,6 +239,11 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
       for (int i = 0; i < NPV; ++i) dst[i] = x[i];
     }
   }
+  if constexpr (FST) {
+#pragma unroll
+    for (int it = 0; it < IFS; ++it)
+      if (lane + 64 * it < NFB) sX[FOFF + lane + 64 * it] = fr[it];
+  }
   __syncthreads();
 
   // the trace columns -F of this wavefront's RHS tiles
@@ -237,12 +256,17 @@ HDGX_DEV void schur64_body(const SchurArgs& A, const int64_t e, const int part, 
         const int tbg = part * TRW + tb;
         const int row = 16 * rt + 4 * i + g, j = 16 * tbg + c;
         if constexpr (TILE) {
-          const int ri = 4 * rt + i;
-          acc[rt][RT + tb][i] = ri < NU / 4 ? -Fe[(ri * TRH + tbg) * 64 + lane] : 0.0;
+          // F's face block f of trace column j = (f NPF + b) + NDF k: row
+          // va + NPV m at q = loc(va, f) + NPF m, column k + NCU b
+          const int jf = j % NDF, k = j / NDF, f = jf / NPF, b = jf % NPF, va = row % NPV, m = row / NPV;
+          const int la = row < NU && j < NH ? L.sTab[LD::NTAB + va + NPV * f] : 255;
+          const double x = sX[FOFF + (la != 255 ? f * NB * NB + la + NPF * m + NB * (k + NCU * b) : 0)];
+          acc[rt][RT + tb][i] = la != 255 ? -x : 0.0;
         } else {
           acc[rt][RT + tb][i] = (tbg < TRH && row < NU && j < NH) ? -Fe[row + NU * s64_t<C>(j)] : 0.0;
         }
       }
+  if constexpr (FST) __syncthreads();          // sX is the elimination's scratch below
   // ---- D' += B M^-1 C, du's column -= B M^-1 Rq, RHS += B M^-1 E, over
   // k-steps of B's columns whose blocks are nonzero (compile-time masks), A
   // fragments (columns of B) fetched a batch of k-steps ahead
